@@ -1,0 +1,286 @@
+"""flashgemm MI355X benchmark: one JSON line.
+
+Main workload (``value``): the FP32 two-GEMM chain E = (A.B).D with A 8192x256,
+B 256x256, D 256x256, entries uniform [-1, 1) from a seed, the intermediate A.B
+kept on chip. In the reference's orientation (multigemm/src/flashgemm.h:21) that is
+flashgemm_multi_f32f32f32(C, B_ref, N=8192, 2, A1, 256, 256, A2, 256, 256) with
+B_ref = A^T (256 x 8192), A1 = B^T, A2 = D^T, C = E^T. One step = one chain call on
+inputs already resident in HBM. value = 2.147 GFLOP per step / step time, TFLOP/s.
+
+Also reported (``configs``), each at its full size:
+  * sgemm_2048: FP32 GEMM M=N=K=2048 (plain FP32 MFMA and split-bf16 x3);
+  * skinny FP32 GEMMs M=8192,N=64,K=8192 and M=N=64,K=65536 (split-K, HBM-bound);
+  * the INT8 and BF16 3-chains X(16384x512).W1(512x512).W2(512x1024).W3(1024x512);
+  * the FP32 precision study (4-chain, 16384 x 1024->4096->1024->4096->1024, columns
+    scaled log-uniform in [1e-3, 1e3]): plain FP32 MFMA vs split-bf16 x3, relative
+    Frobenius error of each against the FP64 oracle on the GPU;
+  * serving: p50/p99 latency of Poisson chain requests, continuous batching vs one
+    launch per GEMM (when the serving engine is built).
+
+cpu_baseline: the reference itself (oracle/_ref, built from its sources) when this
+host has the AVX-512 features it needs, else the C restatement (oracle/), timed on
+this host's cores on the main workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32": 157.3, "bf16": 2516.6, "int8": 5033.2, "fp64": 78.6}  # dense TFLOP/s (TOPS)
+HBM_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_flags() -> set:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("flags"):
+                return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+class Timer:
+    """HIP events on the stream the kernels are launched on."""
+
+    def __init__(self, torch, stream):
+        self.torch, self.stream = torch, stream
+
+    def run(self, fn, steps: int, warmup: int) -> float:
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(self.stream)
+        for _ in range(steps):
+            fn()
+        e1.record(self.stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps  # ms per call
+
+
+def uniform(torch, shape, gen, dtype=None):
+    x = torch.rand(shape, generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
+    return x if dtype is None else x.to(dtype)
+
+
+def bench_main(torch, fg, timer, stream, steps, warmup):
+    gen = torch.Generator(device="cuda").manual_seed(1)
+    N, K1, M1, M2 = 8192, 256, 256, 256
+    B = uniform(torch, (K1, N), gen)
+    A1 = uniform(torch, (M1, K1), gen)
+    A2 = uniform(torch, (M2, M1), gen)
+    C = torch.empty((M2, N), device="cuda", dtype=torch.float32)
+    fused, tn = fg.chain_plan(fg.DTYPE_F32, N, [M1, M2], [K1, M1])
+
+    def step():
+        fg.chain(fg.DTYPE_F32, C, B, [A1, A2], stream=stream)
+
+    ms = timer.run(step, steps, warmup)
+    flop = 2.0 * N * (M1 * K1 + M2 * M1)
+    byts = 4.0 * (K1 * N + M1 * K1 + M2 * M1 + M2 * N)
+    # correctness of what was timed: against the FP64 chain of the same inputs
+    ref = (A2.double() @ (A1.double() @ B.double()))
+    err = float(torch.linalg.norm(C.double() - ref) / torch.linalg.norm(ref))
+    return dict(ms=ms, flop=flop, bytes=byts, fused=fused, tile_n=tn, rel_err_vs_fp64=err,
+                inputs=(B, A1, A2))
+
+
+def bench_sgemm(torch, fg, timer, stream, M, N, K, mode, steps=20, warmup=3):
+    gen = torch.Generator(device="cuda").manual_seed(M + N + K)
+    A = uniform(torch, (M, K), gen)
+    B = uniform(torch, (K, N), gen)
+    C = torch.empty((M, N), device="cuda", dtype=torch.float32)
+    ms = timer.run(lambda: fg.sgemm(C, A, B, M, N, K, f32_mode=mode, stream=stream), steps, warmup)
+    ref = A.double() @ B.double()
+    err = float(torch.linalg.norm(C.double() - ref) / torch.linalg.norm(ref))
+    flop = 2.0 * M * N * K
+    byts = 4.0 * (M * K + K * N + M * N)
+    return dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / PEAK["fp32"],
+                gbs=byts / ms / 1e6, frac_hbm=byts / ms / 1e6 / HBM_GBS, rel_err_vs_fp64=err)
+
+
+def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
+    """X(16384x512).W1(512x512).W2(512x1024).W3(1024x512) in the reference
+    orientation: B = X^T (512 x 16384), A_i = W_i^T."""
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    N, K1, Ms = 16384, 512, [512, 1024, 512]
+    Ks = [K1] + Ms[:-1]
+    if dtype == fg.DTYPE_INT8:
+        B = torch.randint(-127, 128, (K1, N), generator=gen, device="cuda", dtype=torch.int8)
+        As = [torch.randint(-127, 128, (m, k), generator=gen, device="cuda", dtype=torch.int8) for m, k in zip(Ms, Ks)]
+        C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.int32)
+    else:
+        B = (torch.randn((K1, N), generator=gen, device="cuda") / K1 ** 0.5).to(torch.bfloat16)
+        As = [(torch.randn((m, k), generator=gen, device="cuda") / k ** 0.5).to(torch.bfloat16) for m, k in zip(Ms, Ks)]
+        C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float32)
+    fused, tn = fg.chain_plan(dtype, N, Ms, Ks)
+    ms = timer.run(lambda: fg.chain(dtype, C, B, As, stream=stream), steps, warmup)
+    Cu = torch.empty_like(C)
+    ms_unfused = timer.run(lambda: fg.chain(dtype, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF), stream=stream),
+                           steps, warmup)
+    torch.cuda.synchronize()
+    same = bool(torch.equal(C, Cu)) if dtype == fg.DTYPE_INT8 else \
+        float(torch.linalg.norm((C - Cu).double()) / torch.linalg.norm(Cu.double()))
+    flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
+    peak = PEAK["int8" if dtype == fg.DTYPE_INT8 else "bf16"]
+    return dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / peak, fused=fused, tile_n=tn,
+                ms_one_launch_per_gemm=ms_unfused, fused_vs_unfused=(same if dtype == fg.DTYPE_INT8 else None),
+                fused_vs_unfused_rel=(None if dtype == fg.DTYPE_INT8 else same))
+
+
+def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
+    gen = torch.Generator(device="cuda").manual_seed(4)
+    N, widths = 16384, [1024, 4096, 1024, 4096, 1024]
+
+    def scaled(rows, cols):
+        s = 10.0 ** (torch.rand((1, cols), generator=gen, device="cuda") * 6 - 3)
+        return (torch.randn((rows, cols), generator=gen, device="cuda") * s).contiguous()
+
+    B = scaled(widths[0], N)
+    As = [scaled(widths[i + 1], widths[i]) for i in range(4)]
+    Ms, Ks = widths[1:], widths[:-1]
+    ref = torch.empty((widths[-1], N), device="cuda", dtype=torch.float64)
+    fg.f64_chain(ref, B.double(), [a.double() for a in As], N=N, Ms=Ms, Ks=Ks, stream=stream)
+    out = {}
+    flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
+    for name, mode in (("fp32_mfma", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3)):
+        C = torch.empty((widths[-1], N), device="cuda", dtype=torch.float32)
+        o = fg.opts(f32_mode=mode)
+        ms = timer.run(lambda: fg.chain(fg.DTYPE_F32, C, B, As, o=o, stream=stream), steps, warmup)
+        d2, r2 = fg.error_norms(0, C, ref, C.numel(), stream=stream)
+        out[name] = dict(ms=ms, tflops=flop / ms / 1e9, rel_err_vs_fp64=(d2 / r2) ** 0.5)
+    out["speedup_bf16x3_over_fp32"] = out["fp32_mfma"]["ms"] / out["bf16x3"]["ms"]
+    out["err_ratio_bf16x3_over_fp32"] = out["bf16x3"]["rel_err_vs_fp64"] / max(out["fp32_mfma"]["rel_err_vs_fp64"], 1e-30)
+    return out
+
+
+def cpu_baseline(inputs, budget_s=12.0):
+    """The main workload on this host's cores: the compiled reference when it can
+    run here, else the oracle's C restatement."""
+    import numpy as np
+    B, A1, A2 = [t.cpu().numpy() for t in inputs]
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    flop = 2.0 * B.shape[1] * (A1.shape[0] * A1.shape[1] + A2.shape[0] * A2.shape[1])
+    flags = cpu_flags()
+    from oracle import reflib
+    kind = "reference" if (reflib.available() and "avx512f" in flags) else "port"
+    if kind == "reference":
+        R = reflib.Reference(threads)
+        fn = lambda: R.chain_f32(B, [A1, A2])  # noqa: E731
+    else:
+        import oracle as O
+        fn = lambda: O.chain_f32(B, [A1, A2])  # noqa: E731
+    fn()  # warm-up (the reference's drivers warm up too)
+    calls, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        calls += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or calls >= 200:
+            break
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": flop * calls / el / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": kind,
+            "sample": f"{calls} calls of the full 2-GEMM FP32 chain (N=8192, 256->256->256), "
+                      f"warm-up 1, {el:.1f} s, OMP threads {threads}, CPU '{model}', "
+                      f"avx512f={'avx512f' in flags} avx512_vnni={'avx512_vnni' in flags} "
+                      f"avx512_bf16={'avx512_bf16' in flags}"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--main-only", action="store_true", help="skip the other configs and the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "design-only, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import torch
+    import flashgemm_amd as fg
+    fg.init(0)
+    stream = torch.cuda.current_stream()
+    timer = Timer(torch, stream)
+
+    log("main: FP32 2-chain N=8192 256->256->256")
+    main_r = bench_main(torch, fg, timer, stream, args.steps, args.warmup)
+    ms = main_r["ms"]
+    value = main_r["flop"] / ms / 1e9
+    configs = {}
+    if not args.main_only:
+        for name, fn in (
+            ("sgemm_2048_fp32_mfma", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_MFMA)),
+            ("sgemm_2048_bf16x3", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_BF16X3)),
+            ("sgemm_8192x64x8192", lambda: bench_sgemm(torch, fg, timer, stream, 8192, 64, 8192, fg.F32_MFMA)),
+            ("sgemm_64x64x65536", lambda: bench_sgemm(torch, fg, timer, stream, 64, 64, 65536, fg.F32_MFMA)),
+            ("chain3_int8_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_INT8)),
+            ("chain3_bf16_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_BF16)),
+            ("precision_fp32_4chain", lambda: bench_precision(torch, fg, timer, stream)),
+        ):
+            log(f"config {name}")
+            try:
+                configs[name] = fn()
+            except Exception as e:  # report, keep the main line
+                configs[name] = {"error": f"{type(e).__name__}: {e}"}
+    cpu = None
+    if not args.main_only:
+        log("cpu baseline")
+        try:
+            cpu = cpu_baseline(main_r["inputs"])
+        except Exception as e:
+            cpu = {"error": f"{type(e).__name__}: {e}"}
+
+    line = {
+        "metric": "TFLOP/s+%peak per GEMM chain; serving p50/p99 vs plain kernel at stated error",
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: uniform [-1,1) from a fixed seed (torch.Generator 1), resident in HBM",
+        "config": {"workload": "fp32 2-GEMM chain E=(A.B).D, A 8192x256, B 256x256, D 256x256 "
+                               "(reference orientation N=8192, 256->256->256), intermediate on chip",
+                   "frac_of_fp32_mfma_peak": value / PEAK["fp32"],
+                   "fused": main_r["fused"], "tile_n": main_r["tile_n"],
+                   "rel_err_vs_fp64": main_r["rel_err_vs_fp64"]},
+        "roofline": {"bound": "mfma", "achieved": value, "peak": PEAK["fp32"], "unit": "TFLOP/s",
+                     "frac": value / PEAK["fp32"], "traffic": None,
+                     "kernel": "chain_kernel<K_F32> (one launch per step)",
+                     "algorithmic_bytes_per_launch": main_r["bytes"],
+                     "flop_per_launch": main_r["flop"]},
+        "cpu_baseline": cpu,
+        "configs": configs,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,207 @@
+"""flashgemm for MI355X: Python host mirror of the reference's C API.
+
+Everything here is a thin ctypes layer over ``libflashgemm.so`` (built in-tree by
+``python -m flashgemm_amd.build``); no computation happens in Python. Importing
+works without the library; the first call raises ``RuntimeError`` when the
+library is missing, so no path silently falls back to another implementation.
+
+Two layers, mirroring include/flashgemm.h:
+
+* the reference's entry points with the reference's names and argument order
+  (multigemm/src/flashgemm.h:17-25, singlegemm/FP32_ID23.c:35-62), on host numpy
+  arrays: ``flashgemm_set_thread_num``, ``flashgemm_multi_f32f32f32``,
+  ``flashgemm_multi_bf16bf16f32``, ``flashgemm_multi_int8uint8int32``,
+  ``LibShalom_set_thread_nums``, ``LibShalom_sgemm_mp``;
+* the device API (``chain``, ``sgemm``, ``f64_chain``, ``error_norms`` ...) on
+  device buffers given as torch tensors or raw addresses, enqueued on a stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflashgemm.so")
+
+DTYPE_F32, DTYPE_BF16, DTYPE_INT8 = 0, 1, 2
+F32_MFMA, F32_BF16X3 = 0, 1
+LAYOUT_MATH, LAYOUT_REFERENCE = 0, 1
+FUSE_AUTO, FUSE_OFF, FUSE_ON = -1, 0, 1
+
+STATUS = {0: "FG_OK", -1: "FG_ERR_INVALID", -2: "FG_ERR_DOMAIN", -3: "FG_ERR_NOMEM", -4: "FG_ERR_HIP",
+          -5: "FG_ERR_BUDGET", -6: "FG_ERR_NODEVICE", -7: "FG_ERR_UNSUPPORTED"}
+
+
+class FlashgemmError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class ChainOpts(ctypes.Structure):
+    _fields_ = [("layout", ctypes.c_int), ("ref_threads", ctypes.c_int), ("f32_mode", ctypes.c_int),
+                ("fuse", ctypes.c_int), ("tile_n", ctypes.c_int), ("splits", ctypes.c_int)]
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded libflashgemm.so. Raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m flashgemm_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, l, i = ctypes.c_void_p, ctypes.c_long, ctypes.c_int
+        lp, vpp = ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_void_p)
+        L.fg_chain.argtypes = [i, vp, vp, l, i, vpp, lp, lp, ctypes.POINTER(ChainOpts), vp]
+        L.fg_chain_host.argtypes = [i, vp, vp, l, i, vpp, lp, lp, ctypes.POINTER(ChainOpts)]
+        L.fg_chain_plan.argtypes = [i, l, i, lp, lp, ctypes.POINTER(ChainOpts), ctypes.POINTER(i),
+                                    ctypes.POINTER(i)]
+        L.fg_sgemm.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
+        L.fg_sgemm_host.argtypes = [vp, vp, vp, l, l, l, i]
+        L.fg_f64_chain.argtypes = [vp, vp, l, i, vpp, lp, lp, vp]
+        L.fg_to_f64.argtypes = [i, vp, vp, l, vp]
+        L.fg_error_norms.argtypes = [i, vp, vp, l, ctypes.POINTER(ctypes.c_double), vp]
+        L.fg_init.argtypes = [i]
+        L.fg_synchronize.argtypes = [vp]
+        L.fg_last_error.restype = ctypes.c_char_p
+        L.fg_status_string.restype = ctypes.c_char_p
+        L.flashgemm_set_thread_num.argtypes = [i]
+        L.LibShalom_set_thread_nums.argtypes = [i]
+        L.LibShalom_sgemm_mp.argtypes = [i, i, vp, vp, vp, l, l, l]
+        L.flashgemm_sgemm.argtypes = [i, i, vp, vp, vp, l, l, l]
+        _lib = L
+    return _lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        raise FlashgemmError(status, lib().fg_last_error().decode(errors="replace"))
+
+
+def last_error() -> str:
+    return lib().fg_last_error().decode(errors="replace")
+
+
+def opts(layout=LAYOUT_MATH, ref_threads=1, f32_mode=F32_MFMA, fuse=FUSE_AUTO, tile_n=0, splits=0) -> ChainOpts:
+    return ChainOpts(layout, ref_threads, f32_mode, fuse, tile_n, splits)
+
+
+def _addr(x) -> int:
+    """Device or host address of a torch tensor, numpy array or int."""
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        return x.data_ptr()
+    if hasattr(x, "ctypes"):
+        return x.ctypes.data
+    raise TypeError(f"cannot take the address of {type(x)}")
+
+
+def _stream(stream) -> ctypes.c_void_p:
+    if stream is None:
+        return ctypes.c_void_p(0)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)  # torch.cuda.Stream
+
+
+def _arrays(As, Ms=None, Ks=None):
+    n = len(As)
+    ptrs = (ctypes.c_void_p * n)(*[_addr(a) for a in As])
+    Ms = Ms if Ms is not None else [a.shape[0] for a in As]
+    Ks = Ks if Ks is not None else [a.shape[1] for a in As]
+    return n, ptrs, (ctypes.c_long * n)(*Ms), (ctypes.c_long * n)(*Ks)
+
+
+# ------------------------------------------------------------- device API --
+
+def init(device: int = 0) -> None:
+    _check(lib().fg_init(device))
+
+
+def chain(dtype: int, C, B, As, N: int | None = None, Ms=None, Ks=None, o: ChainOpts | None = None,
+          stream=None) -> None:
+    """C = A_n ... A_1 B on device buffers (reference orientation, row-major)."""
+    n, ptrs, M, K = _arrays(As, Ms, Ks)
+    N = N if N is not None else B.shape[1]
+    _check(lib().fg_chain(dtype, _addr(C), _addr(B), N, n, ptrs, M, K,
+                          ctypes.byref(o) if o is not None else None, _stream(stream)))
+
+
+def chain_host(dtype: int, C, B, As, o: ChainOpts | None = None) -> None:
+    """Blocking chain on host numpy arrays."""
+    n, ptrs, M, K = _arrays(As)
+    _check(lib().fg_chain_host(dtype, _addr(C), _addr(B), B.shape[1], n, ptrs, M, K,
+                               ctypes.byref(o) if o is not None else None))
+
+
+def chain_plan(dtype: int, N: int, Ms, Ks, o: ChainOpts | None = None):
+    n = len(Ms)
+    fused, tn = ctypes.c_int(), ctypes.c_int()
+    _check(lib().fg_chain_plan(dtype, N, n, (ctypes.c_long * n)(*Ms), (ctypes.c_long * n)(*Ks),
+                               ctypes.byref(o) if o is not None else None, ctypes.byref(fused), ctypes.byref(tn)))
+    return bool(fused.value), tn.value
+
+
+def sgemm(C, A, B, M: int, N: int, K: int, f32_mode: int = F32_MFMA, splits: int = 0, stream=None) -> None:
+    _check(lib().fg_sgemm(_addr(C), _addr(A), _addr(B), M, N, K, f32_mode, splits, _stream(stream)))
+
+
+def f64_chain(C, B, As, N: int | None = None, Ms=None, Ks=None, stream=None) -> None:
+    n, ptrs, M, K = _arrays(As, Ms, Ks)
+    N = N if N is not None else B.shape[1]
+    _check(lib().fg_f64_chain(_addr(C), _addr(B), N, n, ptrs, M, K, _stream(stream)))
+
+
+def to_f64(x_kind: int, x, out, n: int, stream=None) -> None:
+    _check(lib().fg_to_f64(x_kind, _addr(x), _addr(out), n, _stream(stream)))
+
+
+def error_norms(x_kind: int, x, ref, n: int, stream=None):
+    """(sum (x-ref)^2, sum ref^2) in fp64; x_kind 0 fp32, 1 bf16, 2 int32, 3 fp64."""
+    out = (ctypes.c_double * 2)()
+    _check(lib().fg_error_norms(x_kind, _addr(x), _addr(ref), n, out, _stream(stream)))
+    return out[0], out[1]
+
+
+def synchronize(stream=None) -> None:
+    _check(lib().fg_synchronize(_stream(stream)))
+
+
+# ------------------------------------------------ reference entry points --
+
+def flashgemm_set_thread_num(num: int) -> None:
+    lib().flashgemm_set_thread_num(num)
+
+
+def _legacy(fn, C, B, N, num_gemm, triples):
+    if len(triples) != 3 * num_gemm:
+        raise ValueError("expected num_gemm triples (A_i, M_i, K_i)")
+    args = [ctypes.c_void_p(_addr(C)), ctypes.c_void_p(_addr(B)), ctypes.c_long(N), ctypes.c_int(num_gemm)]
+    for j in range(num_gemm):
+        A, M, K = triples[3 * j:3 * j + 3]
+        args += [ctypes.c_void_p(_addr(A)), ctypes.c_long(M), ctypes.c_long(K)]
+    fn(*args)
+
+
+def flashgemm_multi_f32f32f32(C, B, N: int, num_gemm: int, *triples) -> None:
+    _legacy(lib().flashgemm_multi_f32f32f32, C, B, N, num_gemm, triples)
+
+
+def flashgemm_multi_bf16bf16f32(C, B, N: int, num_gemm: int, *triples) -> None:
+    _legacy(lib().flashgemm_multi_bf16bf16f32, C, B, N, num_gemm, triples)
+
+
+def flashgemm_multi_int8uint8int32(C, B, N: int, num_gemm: int, *triples) -> None:
+    _legacy(lib().flashgemm_multi_int8uint8int32, C, B, N, num_gemm, triples)
+
+
+def LibShalom_set_thread_nums(num: int) -> None:
+    lib().LibShalom_set_thread_nums(num)
+
+
+def LibShalom_sgemm_mp(transa: int, transb: int, C, A, B, M: int, N: int, K: int) -> None:
+    lib().LibShalom_sgemm_mp(transa, transb, _addr(C), _addr(A), _addr(B), M, N, K)

@@ -1,0 +1,515 @@
+// C ABI of libflashgemm: the reference's entry points (part 1 of flashgemm.h) and
+// the status-code API over device pointers (part 2).
+//
+// The reference driver (multigemm/src/flashgemm.c:24-424) partitions N over OpenMP
+// threads, packs A, and runs the chain panel by panel on the host. Here the same
+// call becomes: validate, plan (one fused launch when every intermediate panel fits
+// LDS, otherwise one tiled launch per GEMM with the intermediate in HBM), enqueue on
+// a HIP stream. Host-pointer calls copy in and out around that.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/flashgemm.h"
+#include "fg_internal.h"
+#include "fg_runtime.h"
+
+using namespace fg;
+
+// ------------------------------------------------------------ error state --
+namespace fg {
+static thread_local std::string g_last_error;
+
+int set_error(int status, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return status;
+}
+
+int hip_status(hipError_t e, const char *what) {
+  if (e == hipSuccess) return FG_OK;
+  if (e == hipErrorOutOfMemory) return set_error(FG_ERR_NOMEM, "%s: %s", what, hipGetErrorString(e));
+  return set_error(FG_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------- device ---
+static std::mutex g_mu;
+static int g_device = -1;
+static int g_num_cu = 256;
+
+int ensure_init() {
+  if (g_device >= 0) return FG_OK;
+  return fg_init(0);
+}
+
+int num_cus() { return g_num_cu; }
+
+// ------------------------------------------------------- scratch per stream --
+// Each stream owns named device buffers that only grow. Work on one stream is
+// ordered, so reusing its buffers across calls is safe; different streams never
+// share a buffer.
+struct StreamScratch {
+  std::map<std::string, std::pair<void *, size_t>> bufs;
+};
+static std::map<hipStream_t, StreamScratch> g_scratch;
+
+int scratch(hipStream_t s, const char *name, size_t bytes, void **out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto &slot = g_scratch[s].bufs[name];
+  if (slot.second < bytes) {
+    if (slot.first) {
+      // the old buffer may still be read by queued work on this stream
+      hipError_t e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+      (void)hipFree(slot.first);
+      slot.first = nullptr;
+      slot.second = 0;
+    }
+    size_t cap = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&slot.first, cap);
+    if (e != hipSuccess) return hip_status(e, "hipMalloc(scratch)");
+    slot.second = cap;
+  }
+  *out = slot.first;
+  return FG_OK;
+}
+
+size_t dtype_bytes(int dtype) { return dtype == FG_DTYPE_F32 ? 4 : dtype == FG_DTYPE_BF16 ? 2 : 1; }
+
+int kind_of(int dtype, int f32_mode) {
+  if (dtype == FG_DTYPE_BF16) return K_BF16;
+  if (dtype == FG_DTYPE_INT8) return K_I8;
+  return f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+}
+
+// ----------------------------------------------------------------- planner --
+int plan_splits(long M, long N, long K, int kind) {
+  long tiles = ((M + 127) / 128) * ((N + 127) / 128);
+  if (tiles >= (long)num_cus() * 3 / 4) return 1;
+  long bk = (kind == K_I8) ? 128 : (kind == K_BF16 ? 64 : 32);
+  long want = (num_cus() + tiles - 1) / tiles;
+  long maxs = K / (4 * bk);
+  if (maxs < 1) maxs = 1;
+  long s = want < maxs ? want : maxs;
+  return (int)(s < 1 ? 1 : s);
+}
+
+// Fused tile width: the widest tile that fits LDS and still gives every CU a
+// workgroup; if none gives enough workgroups, the narrowest that fits.
+int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K) {
+  if (num_gemm > chain_fused_max_gemms()) return 0;
+  const int cand[4] = {128, 96, 64, 32};
+  int best = 0;
+  for (int c : cand) {
+    if (chain_fused_lds_bytes(kind, M, K, num_gemm, c) > 160 * 1024) continue;
+    long tiles = (N + c - 1) / c;
+    if (tiles >= num_cus()) return c;
+    best = c;  // keep shrinking for more workgroups
+  }
+  return best;
+}
+
+}  // namespace fg
+
+// ================================================================ part 2 ===
+
+extern "C" {
+
+void fg_chain_opts_default(fg_chain_opts *o) {
+  o->layout = FG_LAYOUT_MATH;
+  o->ref_threads = 1;
+  o->f32_mode = FG_F32_MFMA;
+  o->fuse = FG_FUSE_AUTO;
+  o->tile_n = 0;
+  o->splits = 0;
+}
+
+const char *fg_status_string(int s) {
+  switch (s) {
+    case FG_OK: return "ok";
+    case FG_ERR_INVALID: return "invalid argument";
+    case FG_ERR_DOMAIN: return "reference layout undefined for this shape";
+    case FG_ERR_NOMEM: return "out of device memory";
+    case FG_ERR_HIP: return "HIP error";
+    case FG_ERR_BUDGET: return "over HBM budget";
+    case FG_ERR_NODEVICE: return "no gfx950 device";
+    case FG_ERR_UNSUPPORTED: return "unsupported";
+  }
+  return "unknown status";
+}
+
+const char *fg_last_error(void) { return fg::g_last_error.c_str(); }
+
+int fg_init(int device) {
+  std::lock_guard<std::mutex> lk(fg::g_mu);
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= device || device < 0)
+    return set_error(FG_ERR_NODEVICE, "no HIP device %d (%d visible)", device, n);
+  hipDeviceProp_t p;
+  e = hipGetDeviceProperties(&p, device);
+  if (e != hipSuccess) return hip_status(e, "hipGetDeviceProperties");
+  if (strncmp(p.gcnArchName, "gfx950", 6) != 0)
+    return set_error(FG_ERR_NODEVICE, "device %d is %s, not gfx950", device, p.gcnArchName);
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return hip_status(e, "hipSetDevice");
+  fg::g_device = device;
+  fg::g_num_cu = p.multiProcessorCount;
+  return FG_OK;
+}
+
+int fg_synchronize(void *stream) {
+  return hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+static int validate_chain(int dtype, long N, int num_gemm, const void *const *A, const long *M, const long *K) {
+  if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8) return set_error(FG_ERR_INVALID, "dtype %d", dtype);
+  if (N <= 0) return set_error(FG_ERR_INVALID, "N = %ld", N);
+  if (num_gemm < 1 || num_gemm > 8) return set_error(FG_ERR_INVALID, "num_gemm = %d (1..8)", num_gemm);
+  for (int i = 0; i < num_gemm; ++i) {
+    if (!A || !A[i]) return set_error(FG_ERR_INVALID, "A[%d] is NULL", i);
+    if (M[i] <= 0 || K[i] <= 0) return set_error(FG_ERR_INVALID, "gemm %d: M=%ld K=%ld", i + 1, M[i], K[i]);
+    if (i > 0 && K[i] != M[i - 1])
+      return set_error(FG_ERR_INVALID, "gemm %d: K=%ld but the previous M=%ld", i + 1, K[i], M[i - 1]);
+  }
+  return FG_OK;
+}
+
+// Reference panels for `threads` threads; FG_ERR_DOMAIN when a panel width is not
+// 16 or 32 (outside the reference's domain, where its layout is undefined).
+static int ref_panels(long N, int threads, RefPanels *p) {
+  if (threads < 1) return set_error(FG_ERR_INVALID, "ref_threads = %d", threads);
+  long groups = (N + 15) / 16;
+  long nb = (groups % threads == 0) ? (groups / threads) * 16 : ((groups + threads - 1) / threads) * 16;
+  long ne = N % nb;
+  long num_n = N / nb + (ne == 0 ? 0 : 1);
+  for (long id = 0; id < num_n; ++id) {
+    long NB = (id < num_n - 1) ? nb : (ne > 0 ? ne : nb);
+    long tail = NB % 32;
+    if (tail != 0 && tail != 16)
+      return set_error(FG_ERR_DOMAIN, "reference panel of width %ld (N=%ld, %d threads)", tail, N, threads);
+  }
+  p->nb = nb;
+  p->ne = ne;
+  p->num_n = num_n;
+  p->N = N;
+  return FG_OK;
+}
+
+int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
+                  int *fused, int *tile_n) {
+  fg_chain_opts o;
+  if (opts) o = *opts; else fg_chain_opts_default(&o);
+  int kind = kind_of(dtype, o.f32_mode);
+  bool ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
+  int tn = 0;
+  if (!ref && o.fuse != FG_FUSE_OFF) {
+    if (o.tile_n > 0) {
+      if (chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n) <= 160 * 1024) tn = o.tile_n;
+    } else {
+      tn = plan_tile_n(kind, N, num_gemm, M, K);
+    }
+  }
+  if (o.fuse == FG_FUSE_ON && tn == 0)
+    return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but %s",
+                     ref ? "the reference layout runs unfused" : "no tile width fits LDS");
+  *fused = tn > 0;
+  *tile_n = tn;
+  return FG_OK;
+}
+
+// One GEMM (+ INT8 column sums, + split-K reduction) of the unfused path.
+static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K, int splits,
+                    hipStream_t s) {
+  int *colsum = nullptr;
+  int rc;
+  if (kind == K_I8) {
+    if ((rc = scratch(s, "colsum", sizeof(int) * N, (void **)&colsum))) return rc;
+    if ((rc = hip_status(launch_colsum_i8((const int8_t *)B, K, N, colsum, s), "colsum"))) return rc;
+  }
+  if (splits <= 0) splits = plan_splits(M, N, K, kind);
+  if (splits <= 1) {
+    GemmArgs g{kind, epi, A, B, C, colsum, M, N, K, 1};
+    return hip_status(launch_gemm(g, s), "gemm");
+  }
+  void *part = nullptr;
+  if ((rc = scratch(s, "partials", (size_t)splits * M * N * 4, &part))) return rc;
+  GemmArgs g{kind, EPI_PARTIAL, A, B, part, colsum, M, N, K, splits};
+  if ((rc = hip_status(launch_gemm(g, s), "gemm(split-K)"))) return rc;
+  // the kernel may use fewer slabs than asked when K is short
+  long bk = (kind == K_I8) ? 128 : (kind == K_BF16 ? 64 : 32);
+  long kper = ((K + splits - 1) / splits + bk - 1) / bk * bk;
+  int used = (int)((K + kper - 1) / kper);
+  return hip_status(launch_splitk_reduce(kind, epi, part, used, M, N, colsum, C, s), "split-K reduce");
+}
+
+int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+             const long *K, const fg_chain_opts *opts, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
+  if (!B || !C) return set_error(FG_ERR_INVALID, "B or C is NULL");
+  fg_chain_opts o;
+  if (opts) o = *opts; else fg_chain_opts_default(&o);
+  hipStream_t s = (hipStream_t)stream;
+  int kind = kind_of(dtype, o.f32_mode);
+  bool ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
+  RefPanels panels{};
+  if (ref && (rc = ref_panels(N, o.ref_threads, &panels))) return rc;
+
+  int fused = 0, tn = 0;
+  if ((rc = fg_chain_plan(dtype, N, num_gemm, M, K, &o, &fused, &tn))) return rc;
+  if (fused) {
+    ChainArgs c{};
+    c.kind = kind;
+    c.num_gemm = num_gemm;
+    for (int i = 0; i < num_gemm; ++i) {
+      c.A[i] = A[i];
+      c.M[i] = M[i];
+      c.K[i] = K[i];
+    }
+    c.B = B;
+    c.C = C;
+    c.N = N;
+    c.tile_n = tn;
+    return hip_status(launch_chain_fused(c, s), "fused chain");
+  }
+
+  // Unfused: GEMM i writes its requantized output to HBM; the reference layout adds
+  // a gather that re-reads it the way the reference's next GEMM does.
+  size_t ebytes = (kind == K_I8) ? 1 : (kind == K_BF16 ? 2 : 4);
+  long maxm = 0;
+  for (int i = 0; i + 1 < num_gemm; ++i) maxm = M[i] > maxm ? M[i] : maxm;
+  void *buf[3] = {nullptr, nullptr, nullptr};
+  if (num_gemm > 1) {
+    for (int b = 0; b < (ref ? 3 : 2); ++b) {
+      char name[16];
+      snprintf(name, sizeof name, "spill%d", b);
+      if ((rc = scratch(s, name, (size_t)maxm * N * ebytes, &buf[b]))) return rc;
+    }
+  }
+  const void *X = B;
+  for (int i = 0; i < num_gemm; ++i) {
+    bool last = i == num_gemm - 1;
+    void *Y = last ? C : buf[i & 1];
+    if ((rc = run_gemm(kind, last ? EPI_STORE : EPI_REQUANT, A[i], X, Y, M[i], N, K[i], o.splits, s))) return rc;
+    if (!last && ref) {
+      if ((rc = hip_status(launch_ref_permute(kind == K_I8, Y, buf[2], M[i], panels, s), "ref permute")))
+        return rc;
+      // move the permuted copy into the ping-pong slot the next GEMM reads
+      void *tmp = buf[i & 1];
+      buf[i & 1] = buf[2];
+      buf[2] = tmp;
+      Y = buf[i & 1];
+    }
+    X = Y;
+  }
+  return FG_OK;
+}
+
+int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode, int splits,
+             void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
+    return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
+  int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream);
+}
+
+// -------------------------------------------------------- host-pointer forms --
+static hipStream_t host_stream() {
+  static hipStream_t s = nullptr;
+  static std::once_flag once;
+  std::call_once(once, [] { (void)hipStreamCreate(&s); });
+  return s;
+}
+
+int fg_chain_host(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+                  const long *K, const fg_chain_opts *opts) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
+  if (!B || !C) return set_error(FG_ERR_INVALID, "B or C is NULL");
+  static std::mutex host_mu;  // the host stream's scratch is shared by host calls
+  std::lock_guard<std::mutex> lk(host_mu);
+  hipStream_t s = host_stream();
+  size_t e = dtype_bytes(dtype);
+  void *dB, *dC, *dA[8];
+  if ((rc = scratch(s, "host_B", (size_t)K[0] * N * e, &dB))) return rc;
+  if ((rc = scratch(s, "host_C", (size_t)M[num_gemm - 1] * N * 4, &dC))) return rc;
+  for (int i = 0; i < num_gemm; ++i) {
+    char name[16];
+    snprintf(name, sizeof name, "host_A%d", i);
+    if ((rc = scratch(s, name, (size_t)M[i] * K[i] * e, &dA[i]))) return rc;
+    if ((rc = hip_status(hipMemcpyAsync(dA[i], A[i], (size_t)M[i] * K[i] * e, hipMemcpyHostToDevice, s), "H2D A")))
+      return rc;
+  }
+  if ((rc = hip_status(hipMemcpyAsync(dB, B, (size_t)K[0] * N * e, hipMemcpyHostToDevice, s), "H2D B"))) return rc;
+  if ((rc = fg_chain(dtype, dC, dB, N, num_gemm, (const void *const *)dA, M, K, opts, s))) return rc;
+  if ((rc = hip_status(hipMemcpyAsync(C, dC, (size_t)M[num_gemm - 1] * N * 4, hipMemcpyDeviceToHost, s), "D2H C")))
+    return rc;
+  return hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+int fg_sgemm_host(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode) {
+  const void *As[1] = {A};
+  long Ms[1] = {M}, Ks[1] = {K};
+  fg_chain_opts o;
+  fg_chain_opts_default(&o);
+  o.f32_mode = f32_mode;
+  o.fuse = FG_FUSE_OFF;  // a single GEMM is the tiled kernel (split-K when skinny)
+  return fg_chain_host(FG_DTYPE_F32, C, B, N, 1, As, Ms, Ks, &o);
+}
+
+// ------------------------------------------------------------- FP64 oracle --
+int fg_f64_chain(double *C, const double *B, long N, int num_gemm, const double *const *A, const long *M,
+                 const long *K, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if ((rc = validate_chain(FG_DTYPE_F32, N, num_gemm, (const void *const *)A, M, K))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  long maxm = 0;
+  for (int i = 0; i + 1 < num_gemm; ++i) maxm = M[i] > maxm ? M[i] : maxm;
+  void *buf[2] = {nullptr, nullptr};
+  if (num_gemm > 1)
+    for (int b = 0; b < 2; ++b)
+      if ((rc = scratch(s, b ? "f64_1" : "f64_0", (size_t)maxm * N * 8, &buf[b]))) return rc;
+  const double *X = B;
+  for (int i = 0; i < num_gemm; ++i) {
+    double *Y = (i == num_gemm - 1) ? C : (double *)buf[i & 1];
+    if ((rc = hip_status(launch_f64_gemm(A[i], X, Y, M[i], N, K[i], s), "f64 gemm"))) return rc;
+    X = Y;
+  }
+  return FG_OK;
+}
+
+int fg_to_f64(int x_kind, const void *x, double *out, long n, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (x_kind < 0 || x_kind > 3 || n <= 0) return set_error(FG_ERR_INVALID, "to_f64 kind=%d n=%ld", x_kind, n);
+  return hip_status(launch_to_f64(x_kind, x, out, n, (hipStream_t)stream), "to_f64");
+}
+
+int fg_error_norms(int x_kind, const void *x, const double *ref, long n, double *out, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (x_kind < 0 || x_kind > 3 || n <= 0) return set_error(FG_ERR_INVALID, "error_norms kind=%d n=%ld", x_kind, n);
+  hipStream_t s = (hipStream_t)stream;
+  void *ws;
+  if ((rc = scratch(s, "err", sizeof(double) * (2 + 2 * 1024), &ws))) return rc;
+  if ((rc = hip_status(launch_err_norms(x_kind, x, ref, n, (double *)ws, s), "error norms"))) return rc;
+  if ((rc = hip_status(hipMemcpyAsync(out, ws, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H norms")))
+    return rc;
+  return hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+// ================================================================ part 1 ===
+
+int flashgemm_thread_num = 1;
+
+void flashgemm_set_thread_num(int num) { flashgemm_thread_num = num; }
+
+static int env_f32_mode() {
+  const char *v = getenv("FLASHGEMM_F32_MODE");
+  return (v && strcmp(v, "bf16x3") == 0) ? FG_F32_BF16X3 : FG_F32_MFMA;
+}
+
+static int env_layout() {
+  const char *v = getenv("FLASHGEMM_LAYOUT");
+  return (v && strcmp(v, "math") == 0) ? FG_LAYOUT_MATH : FG_LAYOUT_REFERENCE;
+}
+
+// Reads the variadic triples the way multigemm/src/flashgemm.c:54-68 does, with the
+// same K % 2 check and messages; returns false when the call must stop.
+static bool read_triples(va_list ap, int num_gemm, std::vector<const void *> &A, std::vector<long> &M,
+                         std::vector<long> &K) {
+  if (num_gemm < 1) return false;
+  A.resize(num_gemm);
+  M.resize(num_gemm);
+  K.resize(num_gemm);
+  for (int i = 0; i < num_gemm; ++i) {
+    A[i] = va_arg(ap, const void *);
+    M[i] = va_arg(ap, long);
+    K[i] = va_arg(ap, long);
+    if (K[i] % 2 != 0) {
+      printf("K must be a multiple of 2\n");
+      printf("Dimension error for gemm %d\n", i + 1);
+      fflush(stdout);
+      return false;
+    }
+  }
+  return true;
+}
+
+static void legacy_chain(int dtype, void *C, const void *B, long N, int num_gemm, std::vector<const void *> &A,
+                         std::vector<long> &M, std::vector<long> &K) {
+  fg_chain_opts o;
+  fg_chain_opts_default(&o);
+  o.f32_mode = env_f32_mode();
+  o.layout = (dtype == FG_DTYPE_F32) ? FG_LAYOUT_MATH : env_layout();
+  o.ref_threads = flashgemm_thread_num > 0 ? flashgemm_thread_num : 1;
+  int rc = fg_chain_host(dtype, C, B, N, num_gemm, A.data(), M.data(), K.data(), &o);
+  if (rc == FG_ERR_DOMAIN) {
+    fprintf(stderr, "flashgemm: %s; computing the mathematical chain instead\n", fg_last_error());
+    o.layout = FG_LAYOUT_MATH;
+    rc = fg_chain_host(dtype, C, B, N, num_gemm, A.data(), M.data(), K.data(), &o);
+  }
+  if (rc != FG_OK) fprintf(stderr, "flashgemm: %s (%s)\n", fg_status_string(rc), fg_last_error());
+}
+
+void flashgemm_multi_f32f32f32(float *C, float *B, long N, int num_gemm, ...) {
+  std::vector<const void *> A;
+  std::vector<long> M, K;
+  va_list ap;
+  va_start(ap, num_gemm);
+  bool ok = read_triples(ap, num_gemm, A, M, K);
+  va_end(ap);
+  if (ok) legacy_chain(FG_DTYPE_F32, C, B, N, num_gemm, A, M, K);
+}
+
+void flashgemm_multi_bf16bf16f32(float *C, uint16_t *B, long N, int num_gemm, ...) {
+  std::vector<const void *> A;
+  std::vector<long> M, K;
+  va_list ap;
+  va_start(ap, num_gemm);
+  bool ok = read_triples(ap, num_gemm, A, M, K);
+  va_end(ap);
+  if (ok) legacy_chain(FG_DTYPE_BF16, C, B, N, num_gemm, A, M, K);
+}
+
+void flashgemm_multi_int8uint8int32(int *C, uint8_t *B, long N, int num_gemm, ...) {
+  std::vector<const void *> A;
+  std::vector<long> M, K;
+  va_list ap;
+  va_start(ap, num_gemm);
+  bool ok = read_triples(ap, num_gemm, A, M, K);
+  va_end(ap);
+  if (ok) legacy_chain(FG_DTYPE_INT8, C, B, N, num_gemm, A, M, K);
+}
+
+void LibShalom_set_thread_nums(int num) { (void)num; }
+
+void flashgemm_sgemm(int transa, int transb, float *C, float *A, float *B, long M, long N, long K) {
+  (void)transa;
+  if (transb == 1) return;  // NT: the reference computes nothing (FP32_ID23.c:39-43)
+  int rc = fg_sgemm_host(C, A, B, M, N, K, env_f32_mode());
+  if (rc != FG_OK) fprintf(stderr, "flashgemm: %s (%s)\n", fg_status_string(rc), fg_last_error());
+}
+
+void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, long M, long N, long K) {
+  flashgemm_sgemm(transa, transb, C, A, B, M, N, K);
+}
+
+}  // extern "C"

@@ -1,0 +1,144 @@
+// Internal definitions shared by the flashgemm HIP translation units (gfx950 only).
+//
+// Orientation follows the reference (multigemm/src/flashgemm.h:21-25): a chain
+// computes C = A_n ... A_1 B with B K1 x N, each A_i M_i x K_i, C M_n x N, all
+// row-major with lda = K_i, ldb = ldc = N. N is the wide dimension; the A_i are the
+// weights. Every kernel here produces a (rows of A) x (columns of B) tile.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace fg {
+
+// ----------------------------------------------------------------- dtypes --
+enum Dtype : int { DT_F32 = 0, DT_BF16 = 1, DT_INT8 = 2 };
+
+// How an FP32 problem is computed.
+enum F32Mode : int {
+  F32_MFMA = 0,    // v_mfma_f32_32x32x2_f32, fp32 products and sums
+  F32_BF16X3 = 1,  // x = hi + lo (bf16 pair); Ah.Bh + Ah.Bl + Al.Bh on bf16 MFMA
+};
+
+// What a kernel does with its fp32/int32 accumulator tile.
+enum Epilogue : int {
+  EPI_STORE = 0,    // C (fp32 or int32) row-major
+  EPI_REQUANT = 1,  // next-GEMM operand: fp32 as is, bf16 RNE, int8 saturate
+  EPI_PARTIAL = 2,  // split-K partial (fp32 or int32) into a workspace slab
+};
+
+// Compute "kinds" = (operand storage, MFMA) combinations.
+enum Kind : int { K_F32 = 0, K_BF16 = 1, K_I8 = 2, K_F32X3 = 3 };
+
+typedef __attribute__((ext_vector_type(16))) float f32x16;
+typedef __attribute__((ext_vector_type(16))) int i32x16;
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) int i32x4;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) unsigned u32x4;
+typedef __attribute__((ext_vector_type(2))) unsigned u32x2;
+
+// ------------------------------------------------------------------- bf16 --
+// fp32 -> bf16, round to nearest even; a denormal input becomes a signed zero and
+// NaN stays a quiet NaN. This is the rounding the reference applies to its BF16
+// intermediate (vcvtne2ps2bf16, multigemm/src/kernel_bf16.h:357).
+__host__ __device__ inline uint16_t bf16_rne(float x) {
+  uint32_t u = __builtin_bit_cast(uint32_t, x);
+  if ((u & 0x7f800000u) == 0) u &= 0x80000000u;
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__host__ __device__ inline float bf16_to_f32(uint16_t b) {
+  return __builtin_bit_cast(float, (uint32_t)b << 16);
+}
+// hi/lo split used by the compensated FP32 form: hi = rne(x), lo = rne(x - hi).
+// x - hi is exact in fp32 (Sterbenz), so hi + lo carries 16 significand bits.
+__device__ inline void split_bf16(float x, uint16_t &hi, uint16_t &lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - bf16_to_f32(hi));
+}
+
+__device__ inline int8_t sat_i8(int v) { return (int8_t)(v > 127 ? 127 : (v < -128 ? -128 : v)); }
+
+// ------------------------------------------------------ reference layout --
+// The reference's GEMM i epilogue stores the requantized panel so that GEMM i+1
+// reads X[k][c] = Z[src_row][panel + src_col] (panel-local column c, panel width w):
+//   INT8 w=32: (k&~3)+(c>>3), 4(c&7)+(k&3)      INT8 w=16: (k&~3)+3-(k&3), c
+//   BF16 w=32: (k&~1)+(c>>4), 2(c&15)+(k&1)     BF16 w=16: g+(g+1)/2+(c>>3), 2(c&7)+(k&1), g=k>>1
+// (kernel_int8.h:329-385,1332-1386 against the packed-B order :6-50;
+//  kernel_bf16.h:356-390,1332-1356 against :6-32). Panels start at id*nb + 32j
+// inside each thread's range [id*nb, id*nb+NB) (multigemm/src/flashgemm.c:40-51,109-127).
+struct RefPanels {
+  long nb;     // columns per reference thread
+  long ne;     // N % nb
+  long num_n;  // threads that own columns
+  long N;
+};
+
+__host__ __device__ inline void ref_panel_of(const RefPanels &p, long n, long &p0, long &w) {
+  long id = n / p.nb;
+  long NB = (id < p.num_n - 1) ? p.nb : (p.ne > 0 ? p.ne : p.nb);
+  long j = ((n - id * p.nb) / 32) * 32;
+  p0 = id * p.nb + j;
+  w = (NB - j < 32) ? NB - j : 32;
+}
+
+// Returns false when (w) is outside the reference's domain {16, 32}.
+__host__ __device__ inline bool ref_src(bool is_int8, long w, long k, long c, long &row, long &col) {
+  if (is_int8) {
+    if (w == 32) { row = (k & ~3L) + (c >> 3); col = 4 * (c & 7) + (k & 3); return true; }
+    if (w == 16) { row = (k & ~3L) + 3 - (k & 3); col = c; return true; }
+  } else {
+    if (w == 32) { row = (k & ~1L) + (c >> 4); col = 2 * (c & 15) + (k & 1); return true; }
+    if (w == 16) { long g = k >> 1; row = g + (g + 1) / 2 + (c >> 3); col = 2 * (c & 7) + (k & 1); return true; }
+  }
+  return false;
+}
+
+// ------------------------------------------------------------ launchers ---
+// Launch helpers implemented in the .hip translation units. All return a
+// hipError_t and never synchronize.
+
+struct GemmArgs {
+  int kind;         // Kind
+  int epi;          // Epilogue
+  const void *A;    // M x K row-major (int8 read as u8, bf16 raw bits)
+  const void *B;    // K x N row-major (int8 read as s8)
+  void *C;          // M x N row-major, element type by (kind, epi)
+  const int *colsum;// INT8 only: sum_k B[k][n] for the same K range (int32, N entries)
+  long M, N, K;
+  int splits;       // split-K factor (EPI_PARTIAL writes splits slabs of M x N)
+};
+
+hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
+hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s);
+hipError_t launch_splitk_reduce(int kind, int epi, const void *partials, int splits, long M, long N,
+                                const int *colsum, void *C, hipStream_t s);
+hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, const RefPanels &p,
+                              hipStream_t s);
+
+struct ChainArgs {
+  int kind;                 // Kind
+  int num_gemm;
+  const void *A[8];
+  long M[8], K[8];
+  const void *B;            // K1 x N
+  void *C;                  // M_n x N (fp32 or int32)
+  long N;
+  int tile_n;               // columns per workgroup
+};
+// Fused chain: every intermediate stays in LDS. Returns hipErrorInvalidValue when the
+// largest (K_i + M_i) x tile_n panel does not fit the LDS budget.
+hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s);
+size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n);
+int chain_fused_max_gemms();
+
+// FP64 error oracle (never on the product path).
+hipError_t launch_f64_gemm(const double *A, const double *B, double *C, long M, long N, long K,
+                           hipStream_t s);
+hipError_t launch_to_f64(int src_kind, const void *src, double *dst, long n, hipStream_t s);
+hipError_t launch_err_norms(int x_kind, const void *x, const double *ref, long n, double *out2,
+                            hipStream_t s);
+
+}  // namespace fg

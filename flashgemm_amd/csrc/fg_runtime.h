@@ -1,0 +1,19 @@
+// Host-runtime helpers shared by api.cpp and the grouped / serving translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace fg {
+
+int set_error(int status, const char *fmt, ...);
+int hip_status(hipError_t e, const char *what);
+int ensure_init();
+int num_cus();
+// Named per-stream scratch buffer of at least `bytes` (grow-only).
+int scratch(hipStream_t s, const char *name, size_t bytes, void **out);
+int kind_of(int dtype, int f32_mode);
+int plan_splits(long M, long N, long K, int kind);
+int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K);
+
+}  // namespace fg

@@ -1,0 +1,475 @@
+// Tiled MFMA GEMM for gfx950: C(M x N) = A(M x K) . B(K x N), row-major.
+//
+// Replaces the reference's packing micro-kernels and register-blocked AVX-512
+// kernels (singlegemm/src/kernel.h, kernelMGN.h, PACK_x86.h, PACK_B.h;
+// multigemm/src/kernel_*.h) with one workgroup-level design:
+//   * 256 threads = 4 waves in a 2 x 2 grid, 128 x 128 output tile, each wave
+//     64 x 64 = 2 x 2 MFMA 32x32 blocks;
+//   * A and B staged global -> registers -> LDS, double buffered, in the MFMA
+//     operand layout: 16-byte k-groups per row (A) or per column (B), slot index
+//     g*rows + (r ^ (g & 7)) so both the staging writes and the fragment reads
+//     hit distinct banks;
+//   * kinds: FP32 (v_mfma_f32_32x32x2_f32), BF16 (v_mfma_f32_32x32x16_bf16),
+//     INT8 (v_mfma_i32_32x32x32_i8 with the u8 bias identity
+//     u8(a).b = s8(a ^ 0x80).b + 128.b, the 128.sum_k b term added in the
+//     epilogue), and FP32 split into bf16 hi/lo planes with three products.
+//   * epilogues: store C, requantize into the next GEMM's operand (fp32 / bf16
+//     RNE / int8 saturate), or write a split-K partial slab.
+#include "fg_internal.h"
+
+namespace fg {
+namespace {
+
+constexpr int BM = 128;
+constexpr int BN = 128;
+constexpr int NT = 256;
+
+template <int KIND> struct KT;
+// BK: k per pipeline stage. KG: k per 16-byte LDS group (per plane for F32X3).
+// KC: k consumed per MFMA step (both lane halves). RB/CW: rows x columns of B one
+// thread stages per stage.
+template <> struct KT<K_F32> {
+  static constexpr int BK = 32, KG = 4, ESZ = 4, KC = 8, RB = 4, CW = 4;
+  using Acc = f32x16;
+};
+template <> struct KT<K_BF16> {
+  static constexpr int BK = 64, KG = 8, ESZ = 2, KC = 16, RB = 8, CW = 4;
+  using Acc = f32x16;
+};
+template <> struct KT<K_I8> {
+  static constexpr int BK = 128, KG = 16, ESZ = 1, KC = 32, RB = 16, CW = 4;
+  using Acc = i32x16;
+};
+template <> struct KT<K_F32X3> {
+  static constexpr int BK = 32, KG = 8, ESZ = 4, KC = 16, RB = 8, CW = 2;
+  using Acc = f32x16;
+};
+
+// One operand tile per stage is 16 KB: BK/KG groups x 128 rows x 16 B, or for
+// F32X3 two planes (hi, lo) of BK/8 groups x 128 rows x 16 B.
+constexpr int TILE_U4 = 16384 / 16;
+
+__device__ __forceinline__ int slot(int g, int r) { return g * 128 + (r ^ (g & 7)); }
+
+__device__ __forceinline__ uint32_t pack_bf16x2(uint16_t lo, uint16_t hi) {
+  return (uint32_t)lo | ((uint32_t)hi << 16);
+}
+
+// ---------------------------------------------------------------- staging --
+template <int KIND>
+struct Stager {
+  using T = KT<KIND>;
+  uint32_t a[16];  // 4 x 16 B of A
+  uint32_t b[16];  // RB rows x CW columns of B
+
+  // A tile rows [m0, m0+128) x k [k0, k0+BK): 1024 16-byte units, 4 per thread.
+  __device__ __forceinline__ void load_a(const char *A, long M, long K, long kend, long m0, long k0,
+                                         bool vec, int t) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int u = t + NT * q;
+      int row = u >> 3, v = u & 7;
+      long m = m0 + row;
+      long k = k0 + (long)v * (16 / T::ESZ);
+      uint4 val = {0u, 0u, 0u, 0u};
+      if (m < M) {
+        const char *p = A + ((size_t)m * K + k) * T::ESZ;
+        if (vec && k + 16 / T::ESZ <= kend) {
+          val = *(const uint4 *)p;
+        } else {
+          uint32_t w[4] = {0u, 0u, 0u, 0u};
+          for (int e = 0; e < 16 / T::ESZ; ++e)
+            if (k + e < kend) {
+              uint32_t x;
+              if (T::ESZ == 4) x = *(const uint32_t *)(p + 4 * e);
+              else if (T::ESZ == 2) x = *(const uint16_t *)(p + 2 * e);
+              else x = *(const uint8_t *)(p + e);
+              w[(e * T::ESZ) >> 2] |= x << (((e * T::ESZ) & 3) * 8);
+            }
+          val = {w[0], w[1], w[2], w[3]};
+        }
+      }
+      if (KIND == K_I8) {
+        // u8 -> s8 bias (x ^ 0x80 = x - 128 as signed). Zero padding must stay
+        // zero, so only flip bytes that lie inside the matrix.
+        uint32_t mask[4] = {0u, 0u, 0u, 0u};
+        if (m < M)
+          for (int e = 0; e < 16; ++e)
+            if (k + e < kend) mask[e >> 2] |= 0x80u << ((e & 3) * 8);
+        val.x ^= mask[0]; val.y ^= mask[1]; val.z ^= mask[2]; val.w ^= mask[3];
+      }
+      a[4 * q + 0] = val.x; a[4 * q + 1] = val.y; a[4 * q + 2] = val.z; a[4 * q + 3] = val.w;
+    }
+  }
+
+  __device__ __forceinline__ void store_a(uint4 *sA, int t) const {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      int u = t + NT * q;
+      int row = u >> 3, v = u & 7;
+      if (KIND == K_F32X3) {
+        // 4 floats = k 4v..4v+3 -> group v/2, half v%2 of the hi and lo planes.
+        uint16_t h[4], l[4];
+        for (int e = 0; e < 4; ++e) split_bf16(__builtin_bit_cast(float, a[4 * q + e]), h[e], l[e]);
+        int g = v >> 1, half = v & 1;
+        uint2 *hp = (uint2 *)(sA + slot(g, row)) + half;
+        uint2 *lp = (uint2 *)(sA + TILE_U4 / 2 + slot(g, row)) + half;
+        *hp = make_uint2(pack_bf16x2(h[0], h[1]), pack_bf16x2(h[2], h[3]));
+        *lp = make_uint2(pack_bf16x2(l[0], l[1]), pack_bf16x2(l[2], l[3]));
+      } else {
+        sA[slot(v, row)] = make_uint4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
+      }
+    }
+  }
+
+  // B tile k [k0, k0+BK) x columns [n0, n0+128): thread = RB rows x CW columns.
+  __device__ __forceinline__ void load_b(const char *B, long N, long kend, long n0, long k0, bool vec,
+                                         int t) {
+    constexpr int NCB = 128 / T::CW;  // column blocks
+    int cb = t % NCB, kg = t / NCB;
+    long n = n0 + (long)cb * T::CW;
+    long kb = k0 + (long)kg * T::RB;
+    constexpr int RBYTES = T::CW * T::ESZ;  // bytes per row piece: 16, 8, 4 or 8
+#pragma unroll
+    for (int r = 0; r < T::RB; ++r) {
+      long k = kb + r;
+      const char *p = B + ((size_t)k * N + n) * T::ESZ;
+      bool rowok = k < kend;
+      if (RBYTES == 16) {
+        uint4 v = {0u, 0u, 0u, 0u};
+        if (rowok && vec && n + T::CW <= N) v = *(const uint4 *)p;
+        else if (rowok)
+          for (int c = 0; c < T::CW; ++c)
+            if (n + c < N) (&v.x)[c] = *(const uint32_t *)(p + 4 * c);
+        b[4 * r + 0] = v.x; b[4 * r + 1] = v.y; b[4 * r + 2] = v.z; b[4 * r + 3] = v.w;
+      } else if (RBYTES == 8) {
+        uint2 v = {0u, 0u};
+        if (rowok && vec && n + T::CW <= N) v = *(const uint2 *)p;
+        else if (rowok) {
+          uint32_t w[2] = {0u, 0u};
+          for (int c = 0; c < T::CW; ++c)
+            if (n + c < N) {
+              uint32_t x = (T::ESZ == 4) ? *(const uint32_t *)(p + 4 * c) : *(const uint16_t *)(p + 2 * c);
+              w[(c * T::ESZ) >> 2] |= x << (((c * T::ESZ) & 3) * 8);
+            }
+          v = make_uint2(w[0], w[1]);
+        }
+        b[2 * r + 0] = v.x; b[2 * r + 1] = v.y;
+      } else {
+        uint32_t v = 0u;
+        if (rowok && vec && n + T::CW <= N) v = *(const uint32_t *)p;
+        else if (rowok)
+          for (int c = 0; c < T::CW; ++c)
+            if (n + c < N) v |= (uint32_t)(*(const uint8_t *)(p + c)) << (8 * c);
+        b[r] = v;
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store_b(uint4 *sB, int t) const {
+    constexpr int NCB = 128 / T::CW;
+    int cb = t % NCB, g = t / NCB;
+#pragma unroll
+    for (int c = 0; c < T::CW; ++c) {
+      int n = cb * T::CW + c;
+      if (KIND == K_F32) {
+        sB[slot(g, n)] = make_uint4(b[c], b[4 + c], b[8 + c], b[12 + c]);
+      } else if (KIND == K_BF16) {
+        uint32_t w[4];
+        for (int d = 0; d < 4; ++d) {
+          uint32_t lo = (b[2 * (2 * d) + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
+          uint32_t hi = (b[2 * (2 * d + 1) + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
+          w[d] = lo | (hi << 16);
+        }
+        sB[slot(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
+      } else if (KIND == K_I8) {
+        uint32_t w[4];
+        for (int d = 0; d < 4; ++d) {
+          uint32_t x = 0u;
+          for (int e = 0; e < 4; ++e) x |= ((b[4 * d + e] >> (8 * c)) & 0xffu) << (8 * e);
+          w[d] = x;
+        }
+        sB[slot(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
+      } else {  // F32X3: 8 floats of column c -> hi and lo planes
+        uint16_t h[8], l[8];
+        for (int r = 0; r < 8; ++r) split_bf16(__builtin_bit_cast(float, b[2 * r + c]), h[r], l[r]);
+        sB[slot(g, n)] = make_uint4(pack_bf16x2(h[0], h[1]), pack_bf16x2(h[2], h[3]),
+                                    pack_bf16x2(h[4], h[5]), pack_bf16x2(h[6], h[7]));
+        sB[TILE_U4 / 2 + slot(g, n)] = make_uint4(pack_bf16x2(l[0], l[1]), pack_bf16x2(l[2], l[3]),
+                                                  pack_bf16x2(l[4], l[5]), pack_bf16x2(l[6], l[7]));
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------- MFMA --
+template <int KIND>
+__device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int kc, int wm, int wn,
+                                         int lane, typename KT<KIND>::Acc (&acc)[2][2]) {
+  using T = KT<KIND>;
+  const int h = lane >> 5, l32 = lane & 31;
+  if (KIND == K_F32X3) {
+    const int g = kc / 8 + h;
+    uint4 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      ah[i] = sA[slot(g, wm * 64 + i * 32 + l32)];
+      al[i] = sA[TILE_U4 / 2 + slot(g, wm * 64 + i * 32 + l32)];
+      bh[i] = sB[slot(g, wn * 64 + i * 32 + l32)];
+      bl[i] = sB[TILE_U4 / 2 + slot(g, wn * 64 + i * 32 + l32)];
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        // small terms first, then the dominant hi.hi product
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al[i]),
+                                                            __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[i]),
+                                                            __builtin_bit_cast(bf16x8, bl[j]), acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[i]),
+                                                            __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
+      }
+    return;
+  }
+  const int g = kc / T::KG + h;
+  uint4 a[2], b[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    a[i] = sA[slot(g, wm * 64 + i * 32 + l32)];
+    b[i] = sB[slot(g, wn * 64 + i * 32 + l32)];
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      if (KIND == K_F32) {
+        // lane half h holds k = kc + 4h + s for s = 0..3 in both operands
+        f32x4 av = __builtin_bit_cast(f32x4, a[i]);
+        f32x4 bv = __builtin_bit_cast(f32x4, b[j]);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s], bv[s], acc[i][j], 0, 0, 0);
+      } else if (KIND == K_BF16) {
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[i]),
+                                                            __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
+      } else {
+        acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, a[i]),
+                                                          __builtin_bit_cast(i32x4, b[j]), acc[i][j], 0, 0, 0);
+      }
+    }
+}
+
+// Map a linear workgroup id to (tile_m, tile_n). Consecutive ids are dispatched
+// round-robin over the 8 XCDs; regroup them so that each XCD receives a contiguous
+// run of logical tiles (sharing B column panels in its own L2).
+__device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
+  int total = tiles_m * tiles_n;
+  int L = bid;
+  if ((total & 7) == 0) L = (bid & 7) * (total >> 3) + (bid >> 3);
+  tm = L % tiles_m;
+  tn = L / tiles_m;
+}
+
+template <int KIND, int EPI>
+__global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
+                                                  void *__restrict__ Cv, const int *__restrict__ colsum,
+                                                  long M, long N, long K, long kper, int tiles_m, int tiles_n,
+                                                  int vecA, int vecB) {
+  using T = KT<KIND>;
+  using Acc = typename T::Acc;
+  __shared__ uint4 smem[4 * TILE_U4];  // [stage][A|B]
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  int tm, tn;
+  tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long kbeg = (long)blockIdx.y * kper;
+  const long kend = (kbeg + kper < K) ? kbeg + kper : K;
+
+  Acc acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
+
+  const char *A = (const char *)Av;
+  const char *B = (const char *)Bv;
+  Stager<KIND> st;
+  int nstages = (int)((kend - kbeg + T::BK - 1) / T::BK);
+  if (nstages > 0) {
+    st.load_a(A, M, K, kend, m0, kbeg, vecA, t);
+    st.load_b(B, N, kend, n0, kbeg, vecB, t);
+    st.store_a(smem, t);
+    st.store_b(smem + TILE_U4, t);
+  }
+  __syncthreads();
+  for (int s = 0; s < nstages; ++s) {
+    uint4 *cur = smem + (s & 1) * 2 * TILE_U4;
+    uint4 *nxt = smem + ((s + 1) & 1) * 2 * TILE_U4;
+    const bool more = s + 1 < nstages;
+    if (more) {
+      long k0 = kbeg + (long)(s + 1) * T::BK;
+      st.load_a(A, M, K, kend, m0, k0, vecA, t);
+      st.load_b(B, N, kend, n0, k0, vecB, t);
+    }
+#pragma unroll
+    for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
+    if (more) {
+      st.store_a(nxt, t);
+      st.store_b(nxt + TILE_U4, t);
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column lane%32 per block
+  const int h = lane >> 5, l32 = lane & 31;
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    long n = n0 + wn * 64 + j * 32 + l32;
+    if (n >= N) continue;
+    int bias = 0;
+    if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * colsum[n];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        long m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        if (m >= M) continue;
+        size_t off = (size_t)m * N + n;
+        if (EPI == EPI_PARTIAL) {
+          off += (size_t)blockIdx.y * M * N;
+          if (KIND == K_I8) ((int *)Cv)[off] = (int)acc[i][j][r];
+          else ((float *)Cv)[off] = (float)acc[i][j][r];
+        } else if (KIND == K_I8) {
+          int v = (int)((unsigned)acc[i][j][r] + (unsigned)bias);
+          if (EPI == EPI_STORE) ((int *)Cv)[off] = v;
+          else ((int8_t *)Cv)[off] = sat_i8(v);
+        } else if (KIND == K_BF16 && EPI == EPI_REQUANT) {
+          ((uint16_t *)Cv)[off] = bf16_rne((float)acc[i][j][r]);
+        } else {
+          ((float *)Cv)[off] = (float)acc[i][j][r];
+        }
+      }
+  }
+}
+
+// INT8 column sums sum_k s8(B[k][n]) (exact in int32 for K < 2^24).
+__global__ void colsum_i8_kernel(const int8_t *__restrict__ B, long K, long N, int *__restrict__ out) {
+  long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  int s = 0;
+  for (long k = 0; k < K; ++k) s += B[(size_t)k * N + n];
+  out[n] = s;
+}
+
+// Fixed-order split-K reduction: partial slabs are added in slab order 0..S-1, so
+// the result does not depend on which workgroup finished first.
+template <int KIND, int EPI>
+__global__ void splitk_reduce_kernel(const void *__restrict__ P, int splits, long M, long N,
+                                     const int *__restrict__ colsum, void *__restrict__ C) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t MN = (size_t)M * N;
+  if (e >= MN) return;
+  if (KIND == K_I8) {
+    unsigned s = 0;
+    for (int z = 0; z < splits; ++z) s += (unsigned)((const int *)P)[z * MN + e];
+    s += 128u * (unsigned)colsum[e % N];
+    if (EPI == EPI_STORE) ((int *)C)[e] = (int)s;
+    else ((int8_t *)C)[e] = sat_i8((int)s);
+  } else {
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z) s += ((const float *)P)[z * MN + e];
+    if (KIND == K_BF16 && EPI == EPI_REQUANT) ((uint16_t *)C)[e] = bf16_rne(s);
+    else ((float *)C)[e] = s;
+  }
+}
+
+// Reference-layout re-interpretation of a requantized intermediate Z -> X (gather).
+template <typename E>
+__global__ void ref_permute_kernel(const E *__restrict__ Z, E *__restrict__ X, long rows, RefPanels p,
+                                   bool is_int8) {
+  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)rows * p.N) return;
+  long k = (long)(e / p.N), n = (long)(e % p.N);
+  long p0, w, r, c;
+  ref_panel_of(p, n, p0, w);
+  if (!ref_src(is_int8, w, k, n - p0, r, c) || r >= rows) { X[e] = (E)0; return; }
+  X[e] = Z[(size_t)r * p.N + p0 + c];
+}
+
+template <int KIND, int EPI>
+hipError_t launch_one(const GemmArgs &g, hipStream_t s) {
+  using T = KT<KIND>;
+  int tiles_m = (int)((g.M + BM - 1) / BM), tiles_n = (int)((g.N + BN - 1) / BN);
+  int splits = g.splits > 0 ? g.splits : 1;
+  long kper = ((g.K + splits - 1) / splits + T::BK - 1) / T::BK * T::BK;
+  splits = (int)((g.K + kper - 1) / kper);
+  int esz = T::ESZ;
+  bool vecA = ((g.K * esz) % 16 == 0) && (((uintptr_t)g.A & 15) == 0);
+  bool vecB = (g.N % T::CW == 0) && (((uintptr_t)g.B & (T::CW * esz - 1)) == 0);
+  dim3 grid(tiles_m * tiles_n, splits);
+  hipLaunchKernelGGL((gemm_kernel<KIND, EPI>), grid, dim3(NT), 0, s, g.A, g.B, g.C, g.colsum, g.M, g.N, g.K,
+                     kper, tiles_m, tiles_n, (int)vecA, (int)vecB);
+  return hipGetLastError();
+}
+
+template <int KIND>
+hipError_t launch_kind(const GemmArgs &g, hipStream_t s) {
+  switch (g.epi) {
+    case EPI_STORE: return launch_one<KIND, EPI_STORE>(g, s);
+    case EPI_REQUANT: return launch_one<KIND, EPI_REQUANT>(g, s);
+    case EPI_PARTIAL: return launch_one<KIND, EPI_PARTIAL>(g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
+  if (g.M <= 0 || g.N <= 0 || g.K <= 0) return hipErrorInvalidValue;
+  switch (g.kind) {
+    case K_F32: return launch_kind<K_F32>(g, s);
+    case K_BF16: return launch_kind<K_BF16>(g, s);
+    case K_I8: return launch_kind<K_I8>(g, s);
+    case K_F32X3: return launch_kind<K_F32X3>(g, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s) {
+  hipLaunchKernelGGL(colsum_i8_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, B, K, N, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_splitk_reduce(int kind, int epi, const void *P, int splits, long M, long N,
+                                const int *colsum, void *C, hipStream_t s) {
+  size_t MN = (size_t)M * N;
+  dim3 grid((unsigned)((MN + 255) / 256)), block(256);
+#define FG_RED(K_, E_) hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_>), grid, block, 0, s, P, splits, M, N, colsum, C)
+  if (kind == K_I8) {
+    if (epi == EPI_STORE) FG_RED(K_I8, EPI_STORE); else FG_RED(K_I8, EPI_REQUANT);
+  } else if (kind == K_BF16) {
+    if (epi == EPI_STORE) FG_RED(K_BF16, EPI_STORE); else FG_RED(K_BF16, EPI_REQUANT);
+  } else {
+    FG_RED(K_F32, EPI_STORE);
+  }
+#undef FG_RED
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, const RefPanels &p,
+                              hipStream_t s) {
+  size_t n = (size_t)rows * p.N;
+  dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  if (is_int8)
+    hipLaunchKernelGGL((ref_permute_kernel<int8_t>), grid, block, 0, s, (const int8_t *)Z, (int8_t *)X, rows, p, true);
+  else
+    hipLaunchKernelGGL((ref_permute_kernel<uint16_t>), grid, block, 0, s, (const uint16_t *)Z, (uint16_t *)X, rows, p,
+                       false);
+  return hipGetLastError();
+}
+
+}  // namespace fg

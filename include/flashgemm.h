@@ -1,0 +1,133 @@
+/*
+ * flashgemm.h - C ABI of libflashgemm (MI355X / gfx950, HIP).
+ *
+ * Part 1 keeps the reference's entry points with their exact names, signatures,
+ * argument meaning and orientation, so code written against the reference links
+ * against libflashgemm unchanged. Part 2 is the status-returning API over device
+ * pointers and HIP streams that part 1 is built on.
+ *
+ * Orientation (all parts): a chain computes C = A_n ... A_2 A_1 B, B is K_1 x N,
+ * A_i is M_i x K_i with K_{i+1} = M_i, C is M_n x N; everything row-major with
+ * lda = K_i and ldb = ldc = N. N is the wide dimension. INT8 chains read A bytes as
+ * unsigned and B bytes as signed, accumulate in int32 and saturate each
+ * intermediate to int8; BF16 chains accumulate in fp32 and round each
+ * intermediate to bf16 (round to nearest even); FP32 intermediates stay fp32.
+ *
+ * No torch or HIP types appear in the signatures: device pointers are plain
+ * pointers and a stream is a `void *` holding a hipStream_t (NULL = default).
+ */
+#ifndef FLASHGEMM_H
+#define FLASHGEMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================== 1. reference ABI === */
+
+/* Replaces multigemm/src/flashgemm.h:17 / flashgemm.c:6. Here it is the thread
+ * count whose column-panel decomposition the REFERENCE intermediate layout follows
+ * (see fg_chain_opts.layout); it does not start CPU threads. Default 1. */
+extern int flashgemm_thread_num;
+
+/* Replaces multigemm/src/flashgemm.h:19 (flashgemm.c:8-11). */
+void flashgemm_set_thread_num(int num);
+
+/* Replace multigemm/src/flashgemm.h:21, :23, :25 (flashgemm.c:24-424).
+ * Host pointers; blocking. The variadic tail is num_gemm triples
+ * (T *A_i, long M_i, long K_i), read as in flashgemm.c:54-62.
+ * Like the reference, a K_i that is not a multiple of 2 prints
+ * "K must be a multiple of 2" / "Dimension error for gemm i" and returns with C
+ * untouched. INT8 and BF16 intermediates follow the reference's layout for
+ * flashgemm_thread_num threads unless FLASHGEMM_LAYOUT=math is set; FP32 uses the
+ * FP32 MFMA unless FLASHGEMM_F32_MODE=bf16x3 is set. */
+void flashgemm_multi_f32f32f32(float *C, float *B, long N, int num_gemm, ...);
+void flashgemm_multi_bf16bf16f32(float *C, uint16_t *B, long N, int num_gemm, ...);
+void flashgemm_multi_int8uint8int32(int *C, uint8_t *B, long N, int num_gemm, ...);
+
+/* Replaces singlegemm/src/set_TmTn.h:61-74. Accepted for compatibility; the GPU
+ * needs no thread grid. */
+void LibShalom_set_thread_nums(int num);
+
+/* Replaces the singlegemm dispatcher singlegemm/FP32_ID23.c:35-62 (defined in the
+ * reference's driver): C = A.B, FP32, A M x K, B K x N, C M x N, host pointers,
+ * blocking. transb == 1 (NT) computes nothing, as in the reference. */
+void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, long M, long N, long K);
+/* Same call under this library's own name. */
+void flashgemm_sgemm(int transa, int transb, float *C, float *A, float *B, long M, long N, long K);
+
+/* ================================================= 2. status-code API === */
+
+enum fg_status {
+  FG_OK = 0,
+  FG_ERR_INVALID = -1,    /* bad argument or shape */
+  FG_ERR_DOMAIN = -2,     /* reference layout undefined for this N / thread count */
+  FG_ERR_NOMEM = -3,      /* device allocation failed */
+  FG_ERR_HIP = -4,        /* a HIP call failed (fg_last_error() has the text) */
+  FG_ERR_BUDGET = -5,     /* request exceeds the serving HBM budget */
+  FG_ERR_NODEVICE = -6,   /* no usable gfx950 device */
+  FG_ERR_UNSUPPORTED = -7 /* option combination not implemented */
+};
+
+enum fg_dtype { FG_DTYPE_F32 = 0, FG_DTYPE_BF16 = 1, FG_DTYPE_INT8 = 2 };
+enum fg_f32_mode { FG_F32_MFMA = 0, FG_F32_BF16X3 = 1 };
+enum fg_layout { FG_LAYOUT_MATH = 0, FG_LAYOUT_REFERENCE = 1 };
+enum fg_fuse { FG_FUSE_AUTO = -1, FG_FUSE_OFF = 0, FG_FUSE_ON = 1 };
+
+typedef struct fg_chain_opts {
+  int layout;      /* FG_LAYOUT_MATH (default) or FG_LAYOUT_REFERENCE */
+  int ref_threads; /* thread count the REFERENCE layout follows (>= 1)   */
+  int f32_mode;    /* FG_F32_MFMA (default) or FG_F32_BF16X3            */
+  int fuse;        /* FG_FUSE_AUTO (default), FG_FUSE_OFF, FG_FUSE_ON    */
+  int tile_n;      /* fused tile width: 0 = planner, or 32, 64, 96, 128  */
+  int splits;      /* split-K of unfused GEMMs: 0 = planner              */
+} fg_chain_opts;
+
+void fg_chain_opts_default(fg_chain_opts *opts);
+const char *fg_status_string(int status);
+/* Text of the last error on the calling thread ("" if none). */
+const char *fg_last_error(void);
+/* Select the device (default 0) and check it is gfx950. Called implicitly. */
+int fg_init(int device);
+int fg_synchronize(void *stream);
+
+/* Chain on device pointers, enqueued on `stream`, asynchronous. A[i] are device
+ * pointers. opts may be NULL (defaults). Library scratch (spill intermediates,
+ * split-K partials, column sums) comes from a per-stream cache owned by the
+ * library; calls on one stream are ordered, calls on different streams use
+ * different scratch. */
+int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+             const long *K, const fg_chain_opts *opts, void *stream);
+
+/* Same with host pointers; blocking (copies in, runs, copies out). */
+int fg_chain_host(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+                  const long *K, const fg_chain_opts *opts);
+
+/* FP32 single GEMM C = A.B on device pointers (f32_mode as in fg_chain_opts);
+ * splits = 0 lets the planner choose split-K. */
+int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode, int splits,
+             void *stream);
+int fg_sgemm_host(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode);
+
+/* Plan inspection: writes the planner's decision for a chain without running it.
+ * fused = 1 if the whole chain runs in one fused launch, tile_n its tile width. */
+int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
+                  int *fused, int *tile_n);
+
+/* ---- FP64 error oracle (measurement only; never on a product path) ---- */
+/* C = A_n ... A_1 B in fp64 on device pointers. */
+int fg_f64_chain(double *C, const double *B, long N, int num_gemm, const double *const *A, const long *M,
+                 const long *K, void *stream);
+/* Convert n elements of x (x_kind: 0 fp32, 1 bf16, 2 int32, 3 fp64) to fp64. */
+int fg_to_f64(int x_kind, const void *x, double *out, long n, void *stream);
+/* out[0] = sum (x - ref)^2, out[1] = sum ref^2 in fp64, fixed reduction order.
+ * out is a host pointer; the call synchronizes the stream. */
+int fg_error_norms(int x_kind, const void *x, const double *ref, long n, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLASHGEMM_H */

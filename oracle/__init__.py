@@ -1,0 +1,159 @@
+"""CPU oracle for flashgemm: ctypes bindings over oracle/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY. Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg import this package, as the checker; the product library
+(``flashgemm_amd``) never does.
+
+Each wrapper takes numpy arrays in the reference orientation (B is K1 x N, each
+A_i is M_i x K_i, row-major) and returns C (M_n x N). The C routines restate the
+reference arithmetic; see oracle/oracle.h for the citations and pins.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+LAYOUT_MATH = 0
+LAYOUT_REFERENCE = 1
+
+_lib = None
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so with oracle/Makefile."""
+    args = ["make", "-s", "-C", HERE, "all"]
+    if force:
+        args.insert(2, "-B")
+    subprocess.run(args, check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, lp, i, l = ctypes.c_void_p, ctypes.POINTER(ctypes.c_long), ctypes.c_int, ctypes.c_long
+        for name in ("oracle_chain_f32", "oracle_chain_f64"):
+            getattr(L, name).argtypes = [vp, vp, l, i, ctypes.POINTER(vp), lp, lp]
+            getattr(L, name).restype = i
+        for name in ("oracle_chain_bf16", "oracle_chain_int8"):
+            getattr(L, name).argtypes = [vp, vp, l, i, ctypes.POINTER(vp), lp, lp, i, i]
+            getattr(L, name).restype = i
+        L.oracle_sgemm.argtypes = [vp, vp, vp, l, l, l]
+        L.oracle_sgemm.restype = i
+        L.oracle_bf16_rne.argtypes = [ctypes.c_float]
+        L.oracle_bf16_rne.restype = ctypes.c_uint16
+        L.oracle_partition.argtypes = [l, i, lp, lp, lp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _chain_args(As):
+    n = len(As)
+    ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in As])
+    Ms = (ctypes.c_long * n)(*[a.shape[0] for a in As])
+    Ks = (ctypes.c_long * n)(*[a.shape[1] for a in As])
+    return n, ptrs, Ms, Ks
+
+
+def _check_rc(rc: int, what: str) -> None:
+    if rc == -1:
+        raise ValueError(f"{what}: invalid shapes")
+    if rc == -2:
+        raise ValueError(f"{what}: outside the reference layout domain (panel width or row group)")
+    if rc != 0:
+        raise RuntimeError(f"{what}: rc={rc}")
+
+
+def chain_f32(B: np.ndarray, As) -> np.ndarray:
+    B = np.ascontiguousarray(B, np.float32)
+    As = [np.ascontiguousarray(a, np.float32) for a in As]
+    N = B.shape[1]
+    C = np.empty((As[-1].shape[0], N), np.float32)
+    n, ptrs, Ms, Ks = _chain_args(As)
+    _check_rc(lib().oracle_chain_f32(_ptr(C), _ptr(B), N, n, ptrs, Ms, Ks), "chain_f32")
+    return C
+
+
+def chain_f64(B: np.ndarray, As) -> np.ndarray:
+    B = np.ascontiguousarray(B, np.float64)
+    As = [np.ascontiguousarray(a, np.float64) for a in As]
+    N = B.shape[1]
+    C = np.empty((As[-1].shape[0], N), np.float64)
+    n, ptrs, Ms, Ks = _chain_args(As)
+    _check_rc(lib().oracle_chain_f64(_ptr(C), _ptr(B), N, n, ptrs, Ms, Ks), "chain_f64")
+    return C
+
+
+def chain_bf16(B: np.ndarray, As, layout: int = LAYOUT_MATH, nthreads: int = 1) -> np.ndarray:
+    """B and A_i hold raw bf16 bits (uint16)."""
+    B = np.ascontiguousarray(B, np.uint16)
+    As = [np.ascontiguousarray(a, np.uint16) for a in As]
+    N = B.shape[1]
+    C = np.empty((As[-1].shape[0], N), np.float32)
+    n, ptrs, Ms, Ks = _chain_args(As)
+    _check_rc(lib().oracle_chain_bf16(_ptr(C), _ptr(B), N, n, ptrs, Ms, Ks, layout, nthreads),
+              "chain_bf16")
+    return C
+
+
+def chain_int8(B: np.ndarray, As, layout: int = LAYOUT_MATH, nthreads: int = 1) -> np.ndarray:
+    """B holds bytes read as signed, A_i bytes read as unsigned (the reference's u8.s8)."""
+    B = np.ascontiguousarray(B).view(np.int8)
+    As = [np.ascontiguousarray(a).view(np.int8) for a in As]
+    N = B.shape[1]
+    C = np.empty((As[-1].shape[0], N), np.int32)
+    n, ptrs, Ms, Ks = _chain_args(As)
+    _check_rc(lib().oracle_chain_int8(_ptr(C), _ptr(B), N, n, ptrs, Ms, Ks, layout, nthreads),
+              "chain_int8")
+    return C
+
+
+def sgemm(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    M, K = A.shape
+    N = B.shape[1]
+    C = np.empty((M, N), np.float32)
+    _check_rc(lib().oracle_sgemm(_ptr(C), _ptr(A), _ptr(B), M, N, K), "sgemm")
+    return C
+
+
+def bf16_rne(x: np.ndarray) -> np.ndarray:
+    """Vectorised restatement of oracle_bf16_rne (RNE, denormal -> signed zero, qNaN)."""
+    u = np.ascontiguousarray(x, np.float32).view(np.uint32).copy()
+    den = (u & 0x7F800000) == 0
+    u[den] &= 0x80000000
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    out = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    out[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
+    return out
+
+
+def bf16_to_f32(b: np.ndarray) -> np.ndarray:
+    return (np.asarray(b, np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def partition(N: int, nthreads: int):
+    nb, ne, num = ctypes.c_long(), ctypes.c_long(), ctypes.c_long()
+    lib().oracle_partition(N, nthreads, ctypes.byref(nb), ctypes.byref(ne), ctypes.byref(num))
+    return nb.value, ne.value, num.value
+
+
+def rel_fro(x: np.ndarray, ref: np.ndarray) -> float:
+    """Relative Frobenius error ||x - ref||_F / ||ref||_F in float64."""
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.linalg.norm(ref)
+    return float(np.linalg.norm(x - ref) / (den if den > 0 else 1.0))

@@ -43,6 +43,18 @@ class ChainOpts(ctypes.Structure):
                 ("fuse", ctypes.c_int), ("tile_n", ctypes.c_int), ("splits", ctypes.c_int)]
 
 
+class ChainReq(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int), ("f32_mode", ctypes.c_int), ("C", ctypes.c_void_p),
+                ("B", ctypes.c_void_p), ("N", ctypes.c_long), ("num_gemm", ctypes.c_int),
+                ("A", ctypes.c_void_p * 8), ("M", ctypes.c_long * 8), ("K", ctypes.c_long * 8)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
+                ("num_streams", ctypes.c_int), ("max_batch", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("per_gemm_baseline", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -72,6 +84,26 @@ def lib() -> ctypes.CDLL:
         L.LibShalom_set_thread_nums.argtypes = [i]
         L.LibShalom_sgemm_mp.argtypes = [i, i, vp, vp, vp, l, l, l]
         L.flashgemm_sgemm.argtypes = [i, i, vp, vp, vp, l, l, l]
+        L.fg_grouped_chain.argtypes = [i, ctypes.POINTER(ChainReq), vp]
+        L.fg_grouped_fits.argtypes = [ctypes.POINTER(ChainReq)]
+        L.fg_grouped_graph_create.argtypes = [i, i, ctypes.POINTER(vp)]
+        L.fg_grouped_graph_launch.argtypes = [vp, i, ctypes.POINTER(ChainReq), vp]
+        L.fg_grouped_graph_destroy.argtypes = [vp]
+        L.fg_autotune_chain.argtypes = [i, i, l, i, lp, lp, vp, ctypes.POINTER(i), ctypes.POINTER(i),
+                                        ctypes.POINTER(i), ctypes.POINTER(ctypes.c_double)]
+        L.fg_autotune_save.argtypes = [ctypes.c_char_p]
+        L.fg_autotune_load.argtypes = [ctypes.c_char_p]
+        L.fg_server_config_default.argtypes = [ctypes.POINTER(ServerConfig)]
+        L.fg_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        L.fg_server_add_weights.argtypes = [vp, i, i, i, vpp, lp, lp, ctypes.POINTER(i)]
+        L.fg_server_submit.argtypes = [vp, i, l, vp, vp, ctypes.POINTER(l)]
+        L.fg_server_drain.argtypes = [vp, ctypes.c_double]
+        L.fg_server_run_trace.argtypes = [vp, i, ctypes.POINTER(i), lp, vpp, vpp, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.c_double]
+        L.fg_server_latencies.argtypes = [vp, l, l, ctypes.POINTER(ctypes.c_double)]
+        L.fg_server_num_requests.argtypes = [vp]
+        L.fg_server_num_requests.restype = l
+        L.fg_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -205,3 +237,132 @@ def LibShalom_set_thread_nums(num: int) -> None:
 
 def LibShalom_sgemm_mp(transa: int, transb: int, C, A, B, M: int, N: int, K: int) -> None:
     lib().LibShalom_sgemm_mp(transa, transb, _addr(C), _addr(A), _addr(B), M, N, K)
+
+
+# ------------------------------------------------------ grouped launches --
+
+def chain_req(dtype: int, C, B, As, N: int | None = None, f32_mode: int = F32_MFMA, Ms=None, Ks=None) -> ChainReq:
+    r = ChainReq()
+    r.dtype, r.f32_mode = dtype, f32_mode
+    r.C, r.B = _addr(C), _addr(B)
+    r.N = N if N is not None else B.shape[1]
+    r.num_gemm = len(As)
+    Ms = Ms if Ms is not None else [a.shape[0] for a in As]
+    Ks = Ks if Ks is not None else [a.shape[1] for a in As]
+    for i, a in enumerate(As):
+        r.A[i], r.M[i], r.K[i] = _addr(a), Ms[i], Ks[i]
+    return r
+
+
+def grouped_chain(reqs, stream=None) -> None:
+    arr = (ChainReq * len(reqs))(*reqs)
+    _check(lib().fg_grouped_chain(len(reqs), arr, _stream(stream)))
+
+
+def grouped_fits(req: ChainReq) -> bool:
+    rc = lib().fg_grouped_fits(ctypes.byref(req))
+    if rc < 0:
+        _check(rc)
+    return rc == 1
+
+
+class GroupedGraph:
+    """Captured graph of [table upload -> grouped kernel], replayed per batch."""
+
+    def __init__(self, max_reqs: int, max_tiles: int):
+        self.h = ctypes.c_void_p()
+        _check(lib().fg_grouped_graph_create(max_reqs, max_tiles, ctypes.byref(self.h)))
+
+    def launch(self, reqs, stream=None) -> None:
+        arr = (ChainReq * len(reqs))(*reqs)
+        _check(lib().fg_grouped_graph_launch(self.h, len(reqs), arr, _stream(stream)))
+
+    def close(self) -> None:
+        if self.h:
+            lib().fg_grouped_graph_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+
+# --------------------------------------------------------------- autotune --
+
+def autotune_chain(dtype: int, N: int, Ms, Ks, f32_mode: int = F32_MFMA, stream=None):
+    n = len(Ms)
+    f, t, s, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    _check(lib().fg_autotune_chain(dtype, f32_mode, N, n, (ctypes.c_long * n)(*Ms), (ctypes.c_long * n)(*Ks),
+                                   _stream(stream), ctypes.byref(f), ctypes.byref(t), ctypes.byref(s),
+                                   ctypes.byref(ms)))
+    return {"fused": bool(f.value), "tile_n": t.value, "splits": s.value, "ms": ms.value}
+
+
+def autotune_save(path: str) -> None:
+    _check(lib().fg_autotune_save(path.encode()))
+
+
+def autotune_load(path: str) -> None:
+    _check(lib().fg_autotune_load(path.encode()))
+
+
+def autotune_clear() -> None:
+    lib().fg_autotune_clear()
+
+
+def autotune_size() -> int:
+    return lib().fg_autotune_size()
+
+
+# ---------------------------------------------------------------- serving --
+
+class Server:
+    """Continuous-batching request loop over resident weight sets (device buffers)."""
+
+    def __init__(self, budget_bytes: int | None = None, max_inflight: int | None = None, num_streams: int | None = None,
+                 max_batch: int | None = None, use_graph: bool | None = None, per_gemm_baseline: bool = False):
+        cfg = ServerConfig()
+        lib().fg_server_config_default(ctypes.byref(cfg))
+        if budget_bytes is not None:
+            cfg.hbm_budget_bytes = budget_bytes
+        if max_inflight is not None:
+            cfg.max_inflight = max_inflight
+        if num_streams is not None:
+            cfg.num_streams = num_streams
+        if max_batch is not None:
+            cfg.max_batch = max_batch
+        if use_graph is not None:
+            cfg.use_graph = int(use_graph)
+        cfg.per_gemm_baseline = int(per_gemm_baseline)
+        self.h = ctypes.c_void_p()
+        _check(lib().fg_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def add_weights(self, dtype: int, As, f32_mode: int = F32_MFMA) -> int:
+        n, ptrs, M, K = _arrays(As)
+        sid = ctypes.c_int()
+        _check(lib().fg_server_add_weights(self.h, dtype, f32_mode, n, ptrs, M, K, ctypes.byref(sid)))
+        return sid.value
+
+    def submit(self, set_id: int, B, C, N: int | None = None) -> int:
+        rid = ctypes.c_long()
+        _check(lib().fg_server_submit(self.h, set_id, N if N is not None else B.shape[1], _addr(B), _addr(C),
+                                      ctypes.byref(rid)))
+        return rid.value
+
+    def drain(self, timeout_s: float = 600.0) -> None:
+        _check(lib().fg_server_drain(self.h, timeout_s))
+
+    def run_trace(self, set_ids, Ns, Bs, Cs, arrivals, timeout_s: float = 600.0) -> None:
+        n = len(set_ids)
+        _check(lib().fg_server_run_trace(self.h, n, (ctypes.c_int * n)(*set_ids), (ctypes.c_long * n)(*Ns),
+                                         (ctypes.c_void_p * n)(*[_addr(b) for b in Bs]),
+                                         (ctypes.c_void_p * n)(*[_addr(c) for c in Cs]),
+                                         (ctypes.c_double * n)(*arrivals), timeout_s))
+
+    def latencies(self, first: int = 0, n: int | None = None):
+        total = lib().fg_server_num_requests(self.h)
+        n = total - first if n is None else n
+        out = (ctypes.c_double * n)()
+        _check(lib().fg_server_latencies(self.h, first, n, out))
+        return list(out)
+
+    def close(self) -> None:
+        if self.h:
+            lib().fg_server_destroy(self.h)
+            self.h = ctypes.c_void_p()

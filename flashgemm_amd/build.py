@@ -21,7 +21,7 @@ SOURCES = [
     "gemm_kernels.hip",
     "chain_kernel.hip",
     "f64_kernels.hip",
-    "grouped.hip",
+    "grouped.cpp",
     "api.cpp",
     "serving.cpp",
     "autotune.cpp",

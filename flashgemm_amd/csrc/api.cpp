@@ -207,12 +207,25 @@ static int ref_panels(long N, int threads, RefPanels *p) {
   return FG_OK;
 }
 
-int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
-                  int *fused, int *tile_n) {
+// Decision order: explicit options, then the autotune table, then the heuristics
+// (widest tile that fits and still fills the CUs; split-K for skinny GEMMs).
+static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
+                      int *fused, int *tile_n, int *splits) {
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
   int kind = kind_of(dtype, o.f32_mode);
   bool ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
+  *splits = o.splits;
+  if (!ref && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
+    int f, t, sp;
+    if (autotune_lookup(kind, N, num_gemm, M, K, &f, &t, &sp) &&
+        (!f || chain_fused_lds_bytes(kind, M, K, num_gemm, t) <= LDS_BYTES_MAX)) {
+      *fused = f;
+      *tile_n = f ? t : 0;
+      *splits = f ? 0 : sp;
+      return FG_OK;
+    }
+  }
   int tn = 0;
   if (!ref && o.fuse != FG_FUSE_OFF) {
     if (o.tile_n > 0) {
@@ -227,6 +240,12 @@ int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K,
   *fused = tn > 0;
   *tile_n = tn;
   return FG_OK;
+}
+
+int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
+                  int *fused, int *tile_n) {
+  int splits;
+  return plan_chain(dtype, N, num_gemm, M, K, opts, fused, tile_n, &splits);
 }
 
 // One GEMM (+ INT8 column sums, + split-K reduction) of the unfused path.
@@ -269,7 +288,8 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
   if (ref && (rc = ref_panels(N, o.ref_threads, &panels))) return rc;
 
   int fused = 0, tn = 0;
-  if ((rc = fg_chain_plan(dtype, N, num_gemm, M, K, &o, &fused, &tn))) return rc;
+  int splits = 0;
+  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits))) return rc;
   if (fused) {
     ChainArgs c{};
     c.kind = kind;
@@ -303,7 +323,7 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
   for (int i = 0; i < num_gemm; ++i) {
     bool last = i == num_gemm - 1;
     void *Y = last ? C : buf[i & 1];
-    if ((rc = run_gemm(kind, last ? EPI_STORE : EPI_REQUANT, A[i], X, Y, M[i], N, K[i], o.splits, s))) return rc;
+    if ((rc = run_gemm(kind, last ? EPI_STORE : EPI_REQUANT, A[i], X, Y, M[i], N, K[i], splits, s))) return rc;
     if (!last && ref) {
       if ((rc = hip_status(launch_ref_permute(kind == K_I8, Y, buf[2], M[i], panels, s), "ref permute")))
         return rc;

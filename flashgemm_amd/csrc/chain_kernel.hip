@@ -22,22 +22,8 @@ namespace {
 
 constexpr int NW = 8;  // waves per workgroup
 constexpr int NTH = NW * 64;
-constexpr int MAXG = 8;
+constexpr int MAXG = CHAIN_MAXG;
 constexpr size_t LDS_LIMIT = 160 * 1024;
-
-struct ChainParams {
-  const void *A[MAXG];
-  long M[MAXG], K[MAXG];
-  int xbase[MAXG];  // arena row where X_i starts
-  int ybase[MAXG];  // arena row where Y_i is written (i < n-1)
-  int num_gemm;
-  int S;            // arena rows
-  const void *B;
-  void *C;
-  long N;
-  int vecA[MAXG];
-  int vecB;
-};
 
 template <int KIND> struct CT;
 // KG: k per 16-byte LDS group (per plane for F32X3); KC: k per MFMA step;
@@ -261,20 +247,19 @@ __device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int *colsum)
   }
 }
 
+// One workgroup runs the whole chain on columns [n0, n0 + TN) of B.
 template <int KIND, int TNB>
-__global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
+__device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
   constexpr int TN = TNB * 32;
   constexpr int RP = T::RP;
   constexpr int PF = T::PF;
-  extern __shared__ uint4 arena[];
   const int lo_off = (KIND == K_F32X3) ? (P.S / 8) * TN : 0;  // lo plane (F32X3)
   int *colsum = (int *)(arena + (size_t)(P.S / T::KG) * TN * (KIND == K_F32X3 ? 2 : 1));
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
-  const long n0 = (long)blockIdx.x * TN;
   const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
 
   const long K1p = (P.K[0] + 31) / 32 * 32;
@@ -390,6 +375,12 @@ __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   }
 }
 
+template <int KIND, int TNB>
+__global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
+  extern __shared__ uint4 arena[];
+  chain_tile<KIND, TNB>(P, (long)blockIdx.x * (TNB * 32), arena);
+}
+
 constexpr int KG_of(int kind) { return kind == K_F32 ? 4 : kind == K_BF16 ? 8 : kind == K_I8 ? 16 : 8; }
 constexpr int LB_of(int kind) { return kind == K_F32 ? 4 : kind == K_BF16 ? 2 : kind == K_I8 ? 1 : 4; }
 
@@ -418,18 +409,7 @@ template <int KIND, int TNB>
 hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   using T = CT<KIND>;
   ChainParams P{};
-  P.num_gemm = c.num_gemm;
-  for (int i = 0; i < c.num_gemm; ++i) {
-    P.A[i] = c.A[i];
-    P.M[i] = c.M[i];
-    P.K[i] = c.K[i];
-    P.vecA[i] = ((c.K[i] * T::ESZ) % 16 == 0) && (((uintptr_t)c.A[i] & 15) == 0);
-  }
-  P.S = plan_arena(c.M, c.K, c.num_gemm, P.xbase, P.ybase);
-  P.B = c.B;
-  P.C = c.C;
-  P.N = c.N;
-  P.vecB = (c.N % 4 == 0) && (((uintptr_t)c.B & (4 * T::ESZ - 1)) == 0);
+  make_chain_params(c, &P);
   constexpr int TN = TNB * 32;
   size_t lds = (size_t)P.S * TN * T::LB + (KIND == K_I8 ? TN * sizeof(int) : 0);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
@@ -439,6 +419,32 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   dim3 grid((unsigned)((c.N + TN - 1) / TN));
   hipLaunchKernelGGL(kfn, grid, dim3(NTH), lds, s, P);
   return hipGetLastError();
+}
+
+// Tile widths of the grouped kernel, one per kind.
+constexpr int GT_F32 = 1, GT_BF16 = 1, GT_I8 = 2, GT_F32X3 = 1;
+
+// Persistent grouped launch over a packed table (GroupTable header, the request
+// descriptors, the tile list): workgroup b runs tiles b, b + grid, ...; each tile is
+// a whole chain on TN columns of its request. Counts and offsets are read from the
+// table, so one captured graph replays batches of any size.
+__global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restrict__ table) {
+  extern __shared__ uint4 arena[];
+  const GroupTable *hdr = (const GroupTable *)table;
+  const int n = hdr->num_items;
+  const ChainParams *reqs = (const ChainParams *)(table + hdr->reqs_off);
+  const int2 *items = (const int2 *)(table + hdr->items_off);
+  for (int it = blockIdx.x; it < n; it += gridDim.x) {
+    const int2 w = items[it];
+    const ChainParams &P = reqs[w.x];
+    switch (P.kind) {
+      case K_F32: chain_tile<K_F32, GT_F32>(P, (long)w.y * GT_F32 * 32, arena); break;
+      case K_BF16: chain_tile<K_BF16, GT_BF16>(P, (long)w.y * GT_BF16 * 32, arena); break;
+      case K_I8: chain_tile<K_I8, GT_I8>(P, (long)w.y * GT_I8 * 32, arena); break;
+      case K_F32X3: chain_tile<K_F32X3, GT_F32X3>(P, (long)w.y * GT_F32X3 * 32, arena); break;
+    }
+    __syncthreads();  // the next tile restages the arena
+  }
 }
 
 template <int KIND>
@@ -462,6 +468,44 @@ size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gem
 }
 
 int chain_fused_max_gemms() { return MAXG; }
+
+void make_chain_params(const ChainArgs &c, ChainParams *P) {
+  static const int esz[4] = {4, 2, 1, 4};
+  const int e = esz[c.kind];
+  *P = ChainParams{};
+  P->kind = c.kind;
+  P->num_gemm = c.num_gemm;
+  for (int i = 0; i < c.num_gemm; ++i) {
+    P->A[i] = c.A[i];
+    P->M[i] = c.M[i];
+    P->K[i] = c.K[i];
+    P->vecA[i] = ((c.K[i] * e) % 16 == 0) && (((uintptr_t)c.A[i] & 15) == 0);
+  }
+  P->S = plan_arena(c.M, c.K, c.num_gemm, P->xbase, P->ybase);
+  P->B = c.B;
+  P->C = c.C;
+  P->N = c.N;
+  P->vecB = (c.N % 4 == 0) && (((uintptr_t)c.B & (4 * e - 1)) == 0);
+}
+
+int grouped_tile_n(int kind) {
+  switch (kind) {
+    case K_F32: return GT_F32 * 32;
+    case K_BF16: return GT_BF16 * 32;
+    case K_I8: return GT_I8 * 32;
+    case K_F32X3: return GT_F32X3 * 32;
+  }
+  return 0;
+}
+
+hipError_t launch_grouped(const void *table, int grid, size_t lds, hipStream_t s) {
+  if (lds > LDS_LIMIT || grid <= 0) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void *)grouped_chain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(grouped_chain_kernel, dim3(grid), dim3(NTH), lds, s, (const char *)table);
+  return hipGetLastError();
+}
 
 hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s) {
   if (c.num_gemm < 1 || c.num_gemm > MAXG || c.N <= 0) return hipErrorInvalidValue;

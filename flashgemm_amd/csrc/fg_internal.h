@@ -118,21 +118,57 @@ hipError_t launch_splitk_reduce(int kind, int epi, const void *partials, int spl
 hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, const RefPanels &p,
                               hipStream_t s);
 
+constexpr int CHAIN_MAXG = 8;
+
 struct ChainArgs {
   int kind;                 // Kind
   int num_gemm;
-  const void *A[8];
-  long M[8], K[8];
+  const void *A[CHAIN_MAXG];
+  long M[CHAIN_MAXG], K[CHAIN_MAXG];
   const void *B;            // K1 x N
   void *C;                  // M_n x N (fp32 or int32)
   long N;
   int tile_n;               // columns per workgroup
 };
+
+// Device-side description of one chain for the fused kernels (filled on the host
+// by make_chain_params; copied by value or read from a table in device memory).
+struct ChainParams {
+  int kind;                 // Kind
+  const void *A[CHAIN_MAXG];
+  long M[CHAIN_MAXG], K[CHAIN_MAXG];
+  int xbase[CHAIN_MAXG];    // arena row where X_i starts
+  int ybase[CHAIN_MAXG];    // arena row where Y_i is written (i < n-1)
+  int num_gemm;
+  int S;                    // arena rows
+  const void *B;
+  void *C;
+  long N;
+  int vecA[CHAIN_MAXG];
+  int vecB;
+};
+
+void make_chain_params(const ChainArgs &c, ChainParams *P);
+
 // Fused chain: every intermediate stays in LDS. Returns hipErrorInvalidValue when the
 // largest (K_i + M_i) x tile_n panel does not fit the LDS budget.
 hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s);
 size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n);
 int chain_fused_max_gemms();
+constexpr size_t LDS_BYTES_MAX = 160 * 1024;
+
+// Grouped persistent launch. The device table is a GroupTable header followed by
+// num_reqs ChainParams at reqs_off and num_items int2 (request index, tile index) at
+// items_off. `grid` workgroups walk the tile list; tile widths are fixed per kind
+// (grouped_tile_n).
+struct GroupTable {
+  int num_reqs;
+  int num_items;
+  int reqs_off;   // bytes from the table start
+  int items_off;  // bytes from the table start
+};
+int grouped_tile_n(int kind);
+hipError_t launch_grouped(const void *table, int grid, size_t lds, hipStream_t s);
 
 // FP64 error oracle (never on the product path).
 hipError_t launch_f64_gemm(const double *A, const double *B, double *C, long M, long N, long K,

@@ -1,0 +1,290 @@
+// Grouped launches: many chains of uneven shapes and mixed dtypes in one persistent
+// kernel, directly or replayed from a captured hipGraph.
+//
+// The reference runs one chain per call with an OpenMP team (multigemm/src/
+// flashgemm.c:79-148) and has no batching; this is the GPU counterpart of running
+// many such calls at once. Each request's columns are cut into tiles of the
+// grouped tile width of its kind; the tile list is sorted by decreasing work so the
+// static round-robin over workgroups balances, and every tile runs the whole chain
+// on chip. Requests whose intermediates do not fit LDS at that width are not part
+// of the grouped launch (fg_grouped_chain runs them unfused after it).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "../../include/flashgemm.h"
+#include "fg_internal.h"
+#include "fg_runtime.h"
+
+using namespace fg;
+
+namespace {
+
+struct Packed {
+  std::vector<char> bytes;  // GroupTable + ChainParams[] + int2[]
+  size_t lds = 0;
+  int items = 0;
+};
+
+size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// Validate a request and build its descriptor; fits = intermediates fit on chip.
+int describe(const fg_chain_req &r, ChainParams *P, int *tile_n, size_t *lds, bool *fits) {
+  if (r.dtype < FG_DTYPE_F32 || r.dtype > FG_DTYPE_INT8) return set_error(FG_ERR_INVALID, "dtype %d", r.dtype);
+  if (r.num_gemm < 1 || r.num_gemm > CHAIN_MAXG || r.N <= 0 || !r.B || !r.C)
+    return set_error(FG_ERR_INVALID, "request: num_gemm=%d N=%ld", r.num_gemm, r.N);
+  for (int i = 0; i < r.num_gemm; ++i) {
+    if (!r.A[i] || r.M[i] <= 0 || r.K[i] <= 0) return set_error(FG_ERR_INVALID, "request gemm %d", i + 1);
+    if (i > 0 && r.K[i] != r.M[i - 1]) return set_error(FG_ERR_INVALID, "request gemm %d: K != previous M", i + 1);
+  }
+  int kind = kind_of(r.dtype, r.f32_mode);
+  ChainArgs c{};
+  c.kind = kind;
+  c.num_gemm = r.num_gemm;
+  for (int i = 0; i < r.num_gemm; ++i) {
+    c.A[i] = r.A[i];
+    c.M[i] = r.M[i];
+    c.K[i] = r.K[i];
+  }
+  c.B = r.B;
+  c.C = r.C;
+  c.N = r.N;
+  *tile_n = grouped_tile_n(kind);
+  c.tile_n = *tile_n;
+  *lds = chain_fused_lds_bytes(kind, r.M, r.K, r.num_gemm, *tile_n);
+  *fits = *lds <= LDS_BYTES_MAX;
+  make_chain_params(c, P);
+  return FG_OK;
+}
+
+// Pack the requests that fit into a table; `rest` receives the indices of the others.
+int pack(int count, const fg_chain_req *reqs, Packed *out, std::vector<int> *rest) {
+  std::vector<ChainParams> P;
+  struct Item {
+    int req, tile;
+    double cost;
+  };
+  std::vector<Item> items;
+  size_t lds = 0;
+  for (int r = 0; r < count; ++r) {
+    ChainParams p;
+    int tn;
+    size_t l;
+    bool fits;
+    int rc = describe(reqs[r], &p, &tn, &l, &fits);
+    if (rc) return rc;
+    if (!fits) {
+      if (rest) rest->push_back(r);
+      continue;
+    }
+    lds = std::max(lds, l);
+    double per_col = 0;
+    for (int i = 0; i < p.num_gemm; ++i) per_col += (double)p.M[i] * p.K[i];
+    int idx = (int)P.size();
+    P.push_back(p);
+    long tiles = (reqs[r].N + tn - 1) / tn;
+    for (long t = 0; t < tiles; ++t) {
+      long w = std::min<long>(tn, reqs[r].N - t * tn);
+      items.push_back({idx, (int)t, per_col * (double)std::max<long>(w, 32)});
+    }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+  GroupTable h{};
+  h.num_reqs = (int)P.size();
+  h.num_items = (int)items.size();
+  h.reqs_off = (int)align16(sizeof(GroupTable));
+  h.items_off = (int)align16(h.reqs_off + P.size() * sizeof(ChainParams));
+  out->bytes.assign(h.items_off + items.size() * sizeof(int2), 0);
+  memcpy(out->bytes.data(), &h, sizeof h);
+  if (!P.empty()) memcpy(out->bytes.data() + h.reqs_off, P.data(), P.size() * sizeof(ChainParams));
+  int2 *dst = (int2 *)(out->bytes.data() + h.items_off);
+  for (size_t i = 0; i < items.size(); ++i) dst[i] = make_int2(items[i].req, items[i].tile);
+  out->lds = lds;
+  out->items = (int)items.size();
+  return FG_OK;
+}
+
+// Pinned staging + device table, reused once the event recorded after its last use
+// has completed.
+struct Slot {
+  void *host = nullptr;
+  void *dev = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+};
+
+int slot_reserve(Slot &s, size_t bytes) {
+  if (!s.done) {
+    hipError_t e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_status(e, "hipEventCreate");
+  }
+  hipError_t e = hipEventSynchronize(s.done);  // previous use finished
+  if (e != hipSuccess) return hip_status(e, "hipEventSynchronize");
+  if (s.cap >= bytes) return FG_OK;
+  if (s.host) (void)hipHostFree(s.host);
+  if (s.dev) (void)hipFree(s.dev);
+  s.host = s.dev = nullptr;
+  s.cap = 0;
+  size_t cap = std::max<size_t>(bytes, 64 * 1024);
+  if ((e = hipHostMalloc(&s.host, cap, hipHostMallocDefault)) != hipSuccess) return hip_status(e, "hipHostMalloc");
+  if ((e = hipMalloc(&s.dev, cap)) != hipSuccess) return hip_status(e, "hipMalloc(group table)");
+  s.cap = cap;
+  return FG_OK;
+}
+
+constexpr int RING = 4;
+struct StreamRing {
+  Slot slot[RING];
+  int next = 0;
+};
+std::mutex g_ring_mu;
+std::vector<std::pair<hipStream_t, StreamRing *>> g_rings;
+
+StreamRing *ring_for(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_ring_mu);
+  for (auto &p : g_rings)
+    if (p.first == s) return p.second;
+  g_rings.push_back({s, new StreamRing});
+  return g_rings.back().second;
+}
+
+int grouped_grid(int items) { return std::max(1, std::min(items, num_cus())); }
+
+}  // namespace
+
+struct fg_grouped_graph {
+  struct G {
+    Slot slot;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t copy = nullptr;
+  } g[RING];
+  int next = 0;
+  size_t table_cap = 0;
+};
+
+extern "C" {
+
+int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (count < 0 || (count > 0 && !reqs)) return set_error(FG_ERR_INVALID, "count=%d", count);
+  if (count == 0) return FG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Packed pk;
+  std::vector<int> rest;
+  if ((rc = pack(count, reqs, &pk, &rest))) return rc;
+  if (pk.items > 0) {
+    StreamRing *R = ring_for(s);
+    Slot &sl = R->slot[R->next];
+    R->next = (R->next + 1) % RING;
+    if ((rc = slot_reserve(sl, pk.bytes.size()))) return rc;
+    memcpy(sl.host, pk.bytes.data(), pk.bytes.size());
+    if ((rc = hip_status(hipMemcpyAsync(sl.dev, sl.host, pk.bytes.size(), hipMemcpyHostToDevice, s), "H2D table")))
+      return rc;
+    if ((rc = hip_status(launch_grouped(sl.dev, grouped_grid(pk.items), pk.lds, s), "grouped launch"))) return rc;
+    if ((rc = hip_status(hipEventRecord(sl.done, s), "hipEventRecord"))) return rc;
+  }
+  for (int r : rest) {
+    const fg_chain_req &q = reqs[r];
+    fg_chain_opts o;
+    fg_chain_opts_default(&o);
+    o.f32_mode = q.f32_mode;
+    if ((rc = fg_chain(q.dtype, q.C, q.B, q.N, q.num_gemm, q.A, q.M, q.K, &o, stream))) return rc;
+  }
+  return FG_OK;
+}
+
+int fg_grouped_fits(const fg_chain_req *req) {
+  ChainParams p;
+  int tn;
+  size_t l;
+  bool fits;
+  int rc = describe(*req, &p, &tn, &l, &fits);
+  if (rc) return rc;
+  return fits ? 1 : 0;
+}
+
+int fg_grouped_graph_create(int max_reqs, int max_tiles, fg_grouped_graph **out) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (max_reqs < 1 || max_tiles < 1 || !out) return set_error(FG_ERR_INVALID, "graph sizes");
+  auto *G = new fg_grouped_graph;
+  G->table_cap = align16(sizeof(GroupTable)) + align16((size_t)max_reqs * sizeof(ChainParams)) +
+                 (size_t)max_tiles * sizeof(int2);
+  hipStream_t cs;
+  if ((rc = hip_status(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "hipStreamCreate"))) {
+    delete G;
+    return rc;
+  }
+  for (int i = 0; i < RING && rc == FG_OK; ++i) {
+    auto &g = G->g[i];
+    if ((rc = slot_reserve(g.slot, G->table_cap))) break;
+    memset(g.slot.host, 0, g.slot.cap);
+    // capture: table upload, then the grouped kernel (full LDS, one wave of CUs)
+    if ((rc = hip_status(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture"))) break;
+    hipError_t e1 = hipMemcpyAsync(g.slot.dev, g.slot.host, sizeof(GroupTable), hipMemcpyHostToDevice, cs);
+    hipError_t e2 = launch_grouped(g.slot.dev, num_cus(), LDS_BYTES_MAX, cs);
+    hipError_t e3 = hipStreamEndCapture(cs, &g.graph);
+    if ((rc = hip_status(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "capture"))) break;
+    size_t n = 0;
+    if ((rc = hip_status(hipGraphGetNodes(g.graph, nullptr, &n), "hipGraphGetNodes"))) break;
+    std::vector<hipGraphNode_t> nodes(n);
+    if ((rc = hip_status(hipGraphGetNodes(g.graph, nodes.data(), &n), "hipGraphGetNodes"))) break;
+    for (auto nd : nodes) {
+      hipGraphNodeType t;
+      if (hipGraphNodeGetType(nd, &t) == hipSuccess && t == hipGraphNodeTypeMemcpy) g.copy = nd;
+    }
+    if (!g.copy) {
+      rc = set_error(FG_ERR_HIP, "captured graph has no copy node");
+      break;
+    }
+    rc = hip_status(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  }
+  (void)hipStreamDestroy(cs);
+  if (rc) {
+    fg_grouped_graph_destroy(G);
+    return rc;
+  }
+  *out = G;
+  return FG_OK;
+}
+
+int fg_grouped_graph_launch(fg_grouped_graph *G, int count, const fg_chain_req *reqs, void *stream) {
+  int rc;
+  if (!G || count < 0 || (count > 0 && !reqs)) return set_error(FG_ERR_INVALID, "graph launch");
+  Packed pk;
+  std::vector<int> rest;
+  if ((rc = pack(count, reqs, &pk, &rest))) return rc;
+  if (!rest.empty())
+    return set_error(FG_ERR_UNSUPPORTED, "request %d does not fit on chip at the grouped tile width", rest[0]);
+  if (pk.bytes.size() > G->table_cap) return set_error(FG_ERR_INVALID, "batch exceeds the graph's table");
+  auto &g = G->g[G->next];
+  G->next = (G->next + 1) % RING;
+  if ((rc = slot_reserve(g.slot, G->table_cap))) return rc;  // waits for this slot's last replay
+  memcpy(g.slot.host, pk.bytes.data(), pk.bytes.size());
+  if ((rc = hip_status(hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.slot.dev, g.slot.host,
+                                                         pk.bytes.size(), hipMemcpyHostToDevice),
+                       "update copy node")))
+    return rc;
+  if ((rc = hip_status(hipGraphLaunch(g.exec, (hipStream_t)stream), "hipGraphLaunch"))) return rc;
+  return hip_status(hipEventRecord(g.slot.done, (hipStream_t)stream), "hipEventRecord");
+}
+
+void fg_grouped_graph_destroy(fg_grouped_graph *G) {
+  if (!G) return;
+  for (auto &g : G->g) {
+    if (g.slot.done) (void)hipEventSynchronize(g.slot.done);
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (g.graph) (void)hipGraphDestroy(g.graph);
+    if (g.slot.host) (void)hipHostFree(g.slot.host);
+    if (g.slot.dev) (void)hipFree(g.slot.dev);
+    if (g.slot.done) (void)hipEventDestroy(g.slot.done);
+  }
+  delete G;
+}
+
+}  // extern "C"

@@ -19,6 +19,7 @@
 #ifndef FLASHGEMM_H
 #define FLASHGEMM_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -115,6 +116,87 @@ int fg_sgemm_host(float *C, const float *A, const float *B, long M, long N, long
  * fused = 1 if the whole chain runs in one fused launch, tile_n its tile width. */
 int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
                   int *fused, int *tile_n);
+
+/* ---- grouped launches (no reference counterpart: the reference runs one chain
+ * per call). Many chains of uneven shapes and mixed dtypes, mathematical layout,
+ * all pointers device pointers. ---- */
+typedef struct fg_chain_req {
+  int dtype;             /* FG_DTYPE_*                        */
+  int f32_mode;          /* FG_F32_* for FP32 requests        */
+  void *C;               /* M_n x N, fp32 (int32 for INT8)    */
+  const void *B;         /* K_1 x N                           */
+  long N;
+  int num_gemm;          /* 1..8                              */
+  const void *A[8];      /* A_i, M_i x K_i                    */
+  long M[8], K[8];
+} fg_chain_req;
+
+/* All requests in one persistent launch: each request's columns are cut into tiles
+ * (32 wide; 64 for INT8) that run the whole chain on chip. Requests whose
+ * intermediates do not fit on chip at that width run unfused on the same stream
+ * after it. Asynchronous. */
+int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream);
+/* 1 if the request runs on chip in a grouped launch, 0 if not, < 0 on error. */
+int fg_grouped_fits(const fg_chain_req *req);
+
+/* A captured hipGraph of [descriptor-table upload -> grouped kernel], replayed per
+ * batch. Four captured copies rotate so a batch can be prepared while earlier ones
+ * run. Every request of a replayed batch must fit on chip (else
+ * FG_ERR_UNSUPPORTED); at most max_reqs requests and max_tiles tiles. */
+typedef struct fg_grouped_graph fg_grouped_graph;
+int fg_grouped_graph_create(int max_reqs, int max_tiles, fg_grouped_graph **out);
+int fg_grouped_graph_launch(fg_grouped_graph *graph, int count, const fg_chain_req *reqs, void *stream);
+void fg_grouped_graph_destroy(fg_grouped_graph *graph);
+
+/* ---- serving engine (no reference counterpart: the reference is a blocking
+ * library call). Continuous batching of chain requests against resident weight
+ * sets: requests are admitted in arrival order while a slot is free and their
+ * bytes (B + C) plus the registered weights fit the HBM budget; a scheduler thread
+ * packs every admitted request into the next grouped launch on one of num_streams
+ * streams, so work of requests arriving while others run overlaps on the device.
+ * per_gemm_baseline = 1 runs each request alone with one launch per GEMM instead.
+ * All buffers are device pointers owned by the caller. The engine's streams do not
+ * wait for other streams: B must hold its data, and no other work may still write
+ * B or C, when a request is submitted; C is valid once fg_server_drain returns. ---- */
+typedef struct fg_server_config {
+  size_t hbm_budget_bytes; /* default 200 GiB                      */
+  int max_inflight;        /* request slots, default 1024          */
+  int num_streams;         /* 1..8, default 2                      */
+  int max_batch;           /* requests per launch, default 512     */
+  int use_graph;           /* replay captured graphs, default 1    */
+  int per_gemm_baseline;   /* default 0                            */
+} fg_server_config;
+typedef struct fg_server fg_server;
+
+void fg_server_config_default(fg_server_config *cfg);
+int fg_server_create(const fg_server_config *cfg, fg_server **out);
+/* Register a resident weight chain (device pointers A_i, M_i x K_i). */
+int fg_server_add_weights(fg_server *srv, int dtype, int f32_mode, int num_gemm, const void *const *A,
+                          const long *M, const long *K, int *set_id);
+/* Enqueue C = A_n ... A_1 B for weight set `set_id`; B is K_1 x N, C is M_n x N. */
+int fg_server_submit(fg_server *srv, int set_id, long N, const void *B, void *C, long *req_id);
+/* Wait until every submitted request has completed (or the timeout passes). */
+int fg_server_drain(fg_server *srv, double timeout_s);
+/* Submit n requests at the given arrival offsets (seconds from the call), then drain. */
+int fg_server_run_trace(fg_server *srv, int n, const int *set_ids, const long *Ns, const void *const *Bs,
+                        void *const *Cs, const double *arrival_s, double timeout_s);
+/* Latency (completion - arrival, seconds) of requests [first, first + n). */
+int fg_server_latencies(fg_server *srv, long first, long n, double *out);
+long fg_server_num_requests(fg_server *srv);
+void fg_server_destroy(fg_server *srv);
+
+/* ---- autotuner (replaces the reference's static blocking cost model,
+ * singlegemm/src/set_block.h:8-200 and set_TmTn.h:10-58). Times every runnable
+ * plan for one chain shape (fused at each tile width that fits LDS; unfused with
+ * and without split-K), keeps the fastest in a table keyed by (dtype / FP32 mode,
+ * M_i, K_i, floor(log2 N)) that fg_chain's planner consults when opts leave the
+ * choice open. Outputs may be NULL. ---- */
+int fg_autotune_chain(int dtype, int f32_mode, long N, int num_gemm, const long *M, const long *K, void *stream,
+                      int *fused, int *tile_n, int *splits, double *ms);
+int fg_autotune_save(const char *path); /* "flashgemm-autotune v1 gfx950" text table */
+int fg_autotune_load(const char *path); /* merges entries; rejects other versions   */
+int fg_autotune_clear(void);
+int fg_autotune_size(void);
 
 /* ---- FP64 error oracle (measurement only; never on a product path) ---- */
 /* C = A_n ... A_1 B in fp64 on device pointers. */

@@ -203,6 +203,12 @@ def test_bf16_chain_math_layout(gpu, K1, N, Ms, fuse):
 @pytest.mark.parametrize("K1,N,Ms", CHAIN_SHAPES)
 @pytest.mark.parametrize("fuse", [fg.FUSE_ON, fg.FUSE_OFF])
 def test_f32_chain(gpu, K1, N, Ms, fuse):
+    Ks = [K1] + Ms[:-1]
+    if fuse == fg.FUSE_ON and not fg.chain_plan(fg.DTYPE_F32, N, Ms, Ks)[0]:
+        with pytest.raises(fg.FlashgemmError) as e:  # fp32 arena of (K_i + M_i) rows exceeds LDS
+            run_chain(fg.DTYPE_F32, *rand_chain(np.random.default_rng(0), fg.DTYPE_F32, K1, N, Ms), fuse=fuse)
+        assert e.value.status == -7
+        return
     rng = np.random.default_rng(K1 * 5 + N)
     B, As = rand_chain(rng, fg.DTYPE_F32, K1, N, Ms)
     got = run_chain(fg.DTYPE_F32, B, As, fuse=fuse)

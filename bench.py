@@ -190,7 +190,7 @@ def cpu_baseline(inputs, budget_s=12.0):
         fn()
         calls += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or calls >= 200:
+        if el >= budget_s:
             break
     model = ""
     try:
@@ -213,6 +213,8 @@ def main() -> None:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--main-only", action="store_true", help="skip the other configs and the CPU baseline")
+    ap.add_argument("--only", default=None,
+                    help="run just this named config (for profiling), print its result and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
@@ -225,21 +227,29 @@ def main() -> None:
     stream = torch.cuda.current_stream()
     timer = Timer(torch, stream)
 
+    extra = (
+        ("sgemm_2048_fp32_mfma", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_MFMA)),
+        ("sgemm_2048_bf16x3", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_BF16X3)),
+        ("sgemm_8192x64x8192", lambda: bench_sgemm(torch, fg, timer, stream, 8192, 64, 8192, fg.F32_MFMA)),
+        ("sgemm_64x64x65536", lambda: bench_sgemm(torch, fg, timer, stream, 64, 64, 65536, fg.F32_MFMA)),
+        ("chain3_int8_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_INT8)),
+        ("chain3_bf16_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_BF16)),
+        ("precision_fp32_4chain", lambda: bench_precision(torch, fg, timer, stream)),
+    )
+    if args.only:
+        fns = dict(extra)
+        if args.only not in fns:
+            sys.exit(f"unknown config {args.only}; one of {sorted(fns)}")
+        print(json.dumps({args.only: fns[args.only]()}))
+        return
+
     log("main: FP32 2-chain N=8192 256->256->256")
     main_r = bench_main(torch, fg, timer, stream, args.steps, args.warmup)
     ms = main_r["ms"]
     value = main_r["flop"] / ms / 1e9
     configs = {}
     if not args.main_only:
-        for name, fn in (
-            ("sgemm_2048_fp32_mfma", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_MFMA)),
-            ("sgemm_2048_bf16x3", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_BF16X3)),
-            ("sgemm_8192x64x8192", lambda: bench_sgemm(torch, fg, timer, stream, 8192, 64, 8192, fg.F32_MFMA)),
-            ("sgemm_64x64x65536", lambda: bench_sgemm(torch, fg, timer, stream, 64, 64, 65536, fg.F32_MFMA)),
-            ("chain3_int8_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_INT8)),
-            ("chain3_bf16_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_BF16)),
-            ("precision_fp32_4chain", lambda: bench_precision(torch, fg, timer, stream)),
-        ):
+        for name, fn in extra:
             log(f"config {name}")
             try:
                 configs[name] = fn()

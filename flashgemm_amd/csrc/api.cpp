@@ -108,7 +108,7 @@ int plan_splits(long M, long N, long K, int kind) {
 // Fused tile width: the widest tile that fits LDS and still gives every CU a
 // workgroup; if none gives enough workgroups, the narrowest that fits.
 int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K) {
-  if (num_gemm > chain_fused_max_gemms()) return 0;
+  if (num_gemm > chain_fused_max_gemms() || !chain_fused_shape_ok(kind, N, num_gemm, K)) return 0;
   const int cand[4] = {128, 96, 64, 32};
   int best = 0;
   for (int c : cand) {
@@ -219,7 +219,8 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   if (!ref && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
     int f, t, sp;
     if (autotune_lookup(kind, N, num_gemm, M, K, &f, &t, &sp) &&
-        (!f || chain_fused_lds_bytes(kind, M, K, num_gemm, t) <= LDS_BYTES_MAX)) {
+        (!f || (chain_fused_shape_ok(kind, N, num_gemm, K) &&
+                chain_fused_lds_bytes(kind, M, K, num_gemm, t) <= LDS_BYTES_MAX))) {
       *fused = f;
       *tile_n = f ? t : 0;
       *splits = f ? 0 : sp;
@@ -229,7 +230,9 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   int tn = 0;
   if (!ref && o.fuse != FG_FUSE_OFF) {
     if (o.tile_n > 0) {
-      if (chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n) <= 160 * 1024) tn = o.tile_n;
+      if (chain_fused_shape_ok(kind, N, num_gemm, K) &&
+          chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n) <= LDS_BYTES_MAX)
+        tn = o.tile_n;
     } else {
       tn = plan_tile_n(kind, N, num_gemm, M, K);
     }
@@ -303,7 +306,12 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
     c.C = C;
     c.N = N;
     c.tile_n = tn;
-    return hip_status(launch_chain_fused(c, s), "fused chain");
+    ChainParams P;
+    make_chain_params(c, &P);
+    if (chain_fused_aligned(P)) return hip_status(launch_chain_fused(c, s), "fused chain");
+    if (o.fuse == FG_FUSE_ON)
+      return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but A or B is not 16-byte aligned");
+    // misaligned base pointers: run unfused
   }
 
   // Unfused: GEMM i writes its requantized output to HBM; the reference layout adds

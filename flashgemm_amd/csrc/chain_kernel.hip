@@ -42,7 +42,7 @@ template <> struct CT<K_I8> {
   using Acc = i32x16;
 };
 template <> struct CT<K_F32X3> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, LB = 4, RP = 2, PF = 2;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, LB = 4, RP = 1, PF = 2;
   using Acc = f32x16;
 };
 
@@ -52,79 +52,73 @@ __device__ __forceinline__ int cslot(int g, int n) { return g * TN + (n ^ (g & 7
 // One lane's A fragment for one k-step: 16 bytes (32 for F32X3, as fp32).
 template <int KIND> struct AFrag { uint4 v[KIND == K_F32X3 ? 2 : 1]; };
 
-// K tail or unaligned rows: element-wise loads, zero past K. Out of line so the
-// prefetch slots of the fast path stay plain 16-byte loads.
+// One lane's A fragment for k-step `ks`: row `rowp` (a clamped, always valid row
+// pointer) at element k = ks*KC + h*KA. The fused path requires K*ESZ % 16 == 0 and
+// 16-byte aligned A (the planner sends other shapes to the unfused path), so a
+// fragment lies entirely inside or entirely past K. Out-of-range lanes (row >= M,
+// k >= K) read a clamped in-bounds address and are zeroed with a select: no
+// divergent branches in the k loop.
 template <int KIND>
-__device__ __noinline__ AFrag<KIND> load_afrag_slow(const char *p, long K, long k) {
+__device__ __forceinline__ AFrag<KIND> load_afrag(const char *rowp, long K, long k, bool row_ok) {
   using T = CT<KIND>;
   constexpr int NV = KIND == K_F32X3 ? 2 : 1;
+  const bool ok = row_ok && (k < K);
+  const long kc = ok ? k : 0;
+  const char *p = rowp + kc * T::ESZ;
   AFrag<KIND> f;
-  uint32_t w[4 * NV];
-  for (int d = 0; d < 4 * NV; ++d) w[d] = 0u;
-  for (int e = 0; e < T::KA; ++e)
-    if (k + e < K) {
-      uint32_t x;
-      if (T::ESZ == 4) x = *(const uint32_t *)(p + 4 * e);
-      else if (T::ESZ == 2) x = *(const uint16_t *)(p + 2 * e);
-      else x = *(const uint8_t *)(p + e) ^ 0x80u;  // u8 -> s8 bias, only inside K
-      w[(e * T::ESZ) >> 2] |= x << (((e * T::ESZ) & 3) * 8);
-    }
-  for (int q = 0; q < NV; ++q) f.v[q] = make_uint4(w[4 * q], w[4 * q + 1], w[4 * q + 2], w[4 * q + 3]);
-  return f;
-}
-
-template <int KIND>
-__device__ __forceinline__ AFrag<KIND> load_afrag(const char *A, long M, long K, long row, long k, bool vec) {
-  using T = CT<KIND>;
-  AFrag<KIND> f;
-  constexpr int NV = KIND == K_F32X3 ? 2 : 1;
-  if (row >= M) {
 #pragma unroll
-    for (int q = 0; q < NV; ++q) f.v[q] = make_uint4(0u, 0u, 0u, 0u);
-    return f;
-  }
-  const char *p = A + ((size_t)row * K + k) * T::ESZ;
-  if (vec && k + T::KA <= K) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) f.v[q] = *(const uint4 *)(p + 16 * q);
+  for (int q = 0; q < NV; ++q) {
+    uint4 v = *(const uint4 *)(p + 16 * q);
     if (KIND == K_I8) {  // u8 -> s8 bias: x ^ 0x80 = x - 128 as signed
-      f.v[0].x ^= 0x80808080u; f.v[0].y ^= 0x80808080u; f.v[0].z ^= 0x80808080u; f.v[0].w ^= 0x80808080u;
+      v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
     }
-    return f;
+    f.v[q].x = ok ? v.x : 0u;
+    f.v[q].y = ok ? v.y : 0u;
+    f.v[q].z = ok ? v.z : 0u;
+    f.v[q].w = ok ? v.w : 0u;
   }
-  if (k >= K) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) f.v[q] = make_uint4(0u, 0u, 0u, 0u);
-    return f;
-  }
-  return load_afrag_slow<KIND>(p, K, k);
+  return f;
 }
 
 __device__ __forceinline__ uint32_t pk(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
 
+// One lane's B fragments for one k-step: TNB column blocks (hi and lo planes for
+// F32X3), read from the arena one step ahead of their MFMAs.
+template <int KIND, int TNB> struct BFrag {
+  uint4 b[TNB];
+  uint4 lo[KIND == K_F32X3 ? TNB : 1];
+};
+
 template <int KIND, int TNB>
-__device__ __forceinline__ void mma_k(const uint4 *arena, int lo_off, int g, int lane,
-                                      const AFrag<KIND> (&af)[CT<KIND>::RP],
-                                      typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
+__device__ __forceinline__ void read_bfrag(const uint4 *arena, int lo_off, int g, int l32, BFrag<KIND, TNB> &f) {
   constexpr int TN = TNB * 32;
+#pragma unroll
+  for (int j = 0; j < TNB; ++j) {
+    f.b[j] = arena[cslot<TN>(g, j * 32 + l32)];
+    if (KIND == K_F32X3) f.lo[j] = arena[lo_off + cslot<TN>(g, j * 32 + l32)];
+  }
+}
+
+template <int KIND, int TNB>
+__device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KIND> (&af)[CT<KIND>::RP],
+                                      typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
   constexpr int RP = CT<KIND>::RP;
-  const int l32 = lane & 31;
   if (KIND == K_F32X3) {
     uint4 ah[RP], al[RP];
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
-      uint16_t h[8], l[8];
       const uint32_t w[8] = {af[i].v[0].x, af[i].v[0].y, af[i].v[0].z, af[i].v[0].w,
                              af[i].v[1].x, af[i].v[1].y, af[i].v[1].z, af[i].v[1].w};
+      uint32_t h[4], l[4];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) split_bf16(__builtin_bit_cast(float, w[e]), h[e], l[e]);
-      ah[i] = make_uint4(pk(h[0], h[1]), pk(h[2], h[3]), pk(h[4], h[5]), pk(h[6], h[7]));
-      al[i] = make_uint4(pk(l[0], l[1]), pk(l[2], l[3]), pk(l[4], l[5]), pk(l[6], l[7]));
+      for (int e = 0; e < 4; ++e)
+        split_pk(__builtin_bit_cast(float, w[2 * e]), __builtin_bit_cast(float, w[2 * e + 1]), h[e], l[e]);
+      ah[i] = make_uint4(h[0], h[1], h[2], h[3]);
+      al[i] = make_uint4(l[0], l[1], l[2], l[3]);
     }
 #pragma unroll
     for (int j = 0; j < TNB; ++j) {
-      uint4 bh = arena[cslot<TN>(g, j * 32 + l32)];
-      uint4 bl = arena[lo_off + cslot<TN>(g, j * 32 + l32)];
+      const uint4 bh = bf.b[j], bl = bf.lo[j];
 #pragma unroll
       for (int i = 0; i < RP; ++i) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al[i]),
@@ -139,7 +133,7 @@ __device__ __forceinline__ void mma_k(const uint4 *arena, int lo_off, int g, int
   }
 #pragma unroll
   for (int j = 0; j < TNB; ++j) {
-    uint4 b = arena[cslot<TN>(g, j * 32 + l32)];
+    const uint4 b = bf.b[j];
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
       if (KIND == K_F32) {
@@ -160,44 +154,39 @@ __device__ __forceinline__ void mma_k(const uint4 *arena, int lo_off, int g, int
 }
 
 // Stage the B tile (rows [0, K1p) x columns [n0, n0+TN)) into the arena at row 0.
+// The fused path requires N % 4 == 0 and a 4-element aligned B, so each 4-column
+// group lies entirely inside or outside the matrix; out-of-range rows and groups
+// read a clamped address and are zeroed with a select.
 template <int KIND, int TN>
-__device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K1p, long N, long n0, bool vec) {
+__device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K1p, long N, long n0) {
   using T = CT<KIND>;
   // one task = KG rows x 4 columns of one k-group
   const int ngroups = (int)(K1p / T::KG);
   const int tasks = ngroups * (TN / 4);
   for (int task = threadIdx.x; task < tasks; task += NTH) {
     int g = task / (TN / 4), cb = task % (TN / 4);
-    long nb = n0 + cb * 4;
+    const long nb = n0 + cb * 4;
+    const bool nok = nb < N;
     // v[r][c] raw element bits
     uint32_t v[T::KG][4];
 #pragma unroll
     for (int r = 0; r < T::KG; ++r) {
-      long k = (long)g * T::KG + r;
-#pragma unroll
-      for (int c = 0; c < 4; ++c) v[r][c] = 0u;
-      if (k >= K1) continue;
-      const char *p = B + ((size_t)k * N + nb) * T::ESZ;
-      if (vec && nb + 4 <= N) {
-        if (T::ESZ == 4) {
-          uint4 x = *(const uint4 *)p;
-          v[r][0] = x.x; v[r][1] = x.y; v[r][2] = x.z; v[r][3] = x.w;
-        } else if (T::ESZ == 2) {
-          uint2 x = *(const uint2 *)p;
-          v[r][0] = x.x & 0xffffu; v[r][1] = x.x >> 16; v[r][2] = x.y & 0xffffu; v[r][3] = x.y >> 16;
-        } else {
-          uint32_t x = *(const uint32_t *)p;
-#pragma unroll
-          for (int c = 0; c < 4; ++c) v[r][c] = (x >> (8 * c)) & 0xffu;
-        }
+      const long k = (long)g * T::KG + r;
+      const bool ok = nok && k < K1;
+      const char *p = B + ((size_t)(ok ? k : 0) * N + (ok ? nb : 0)) * T::ESZ;
+      if (T::ESZ == 4) {
+        uint4 x = *(const uint4 *)p;
+        v[r][0] = x.x; v[r][1] = x.y; v[r][2] = x.z; v[r][3] = x.w;
+      } else if (T::ESZ == 2) {
+        uint2 x = *(const uint2 *)p;
+        v[r][0] = x.x & 0xffffu; v[r][1] = x.x >> 16; v[r][2] = x.y & 0xffffu; v[r][3] = x.y >> 16;
       } else {
-        for (int c = 0; c < 4; ++c)
-          if (nb + c < N) {
-            if (T::ESZ == 4) v[r][c] = *(const uint32_t *)(p + 4 * c);
-            else if (T::ESZ == 2) v[r][c] = *(const uint16_t *)(p + 2 * c);
-            else v[r][c] = *(const uint8_t *)(p + c);
-          }
+        uint32_t x = *(const uint32_t *)p;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[r][c] = (x >> (8 * c)) & 0xffu;
       }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[r][c] = ok ? v[r][c] : 0u;
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -215,11 +204,12 @@ __device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K
           w[d] = v[4 * d][c] | (v[4 * d + 1][c] << 8) | (v[4 * d + 2][c] << 16) | (v[4 * d + 3][c] << 24);
         arena[sl] = make_uint4(w[0], w[1], w[2], w[3]);
       } else {
-        uint16_t h[8], l[8];
+        uint32_t h[4], l[4];
 #pragma unroll
-        for (int r = 0; r < 8; ++r) split_bf16(__builtin_bit_cast(float, v[r][c]), h[r], l[r]);
-        arena[sl] = make_uint4(pk(h[0], h[1]), pk(h[2], h[3]), pk(h[4], h[5]), pk(h[6], h[7]));
-        arena[lo_off + sl] = make_uint4(pk(l[0], l[1]), pk(l[2], l[3]), pk(l[4], l[5]), pk(l[6], l[7]));
+        for (int r = 0; r < 4; ++r)
+          split_pk(__builtin_bit_cast(float, v[2 * r][c]), __builtin_bit_cast(float, v[2 * r + 1][c]), h[r], l[r]);
+        arena[sl] = make_uint4(h[0], h[1], h[2], h[3]);
+        arena[lo_off + sl] = make_uint4(l[0], l[1], l[2], l[3]);
       }
     }
   }
@@ -263,7 +253,7 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
 
   const long K1p = (P.K[0] + 31) / 32 * 32;
-  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1p, P.N, n0, P.vecB);
+  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1p, P.N, n0);
   __syncthreads();
 
   for (int gi = 0; gi < P.num_gemm; ++gi) {
@@ -272,13 +262,20 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
     const int xg0 = P.xbase[gi] / T::KG;
     const bool last = gi == P.num_gemm - 1;
     const char *A = (const char *)P.A[gi];
-    const bool vec = P.vecA[gi];
     if (KIND == K_I8) {
       colsum_lds<TN>(arena, xg0, (int)((K + 31) / 32 * 32 / T::KG), colsum);
       __syncthreads();
     }
+    // k-steps: `full` ones have every lane's fragment inside K; they run in chunks of
+    // PF with no bounds logic. The remaining steps (K tail) use checked loads.
     const int ksteps = (int)((K + T::KC - 1) / T::KC);
-    const int nchunks = (ksteps + PF - 1) / PF;
+    const int kfull = (int)(K / T::KC);
+    const int nfc = kfull / PF;  // full chunks
+    // INT8: the accumulators start at the bias 128 * sum_k X[k][n] of their column,
+    // so the MFMAs accumulate u8(a).b directly.
+    int bias0[TNB];
+#pragma unroll
+    for (int j = 0; j < TNB; ++j) bias0[j] = (KIND == K_I8) ? 128 * colsum[j * 32 + l32] : 0;
     for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += (long)NW * RP * 32) {
       Acc acc[RP][TNB];
 #pragma unroll
@@ -286,40 +283,77 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
 #pragma unroll
         for (int j = 0; j < TNB; ++j)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
+          for (int r = 0; r < 16; ++r) acc[i][j][r] = bias0[j];
 
+      // Per-lane row pointers clamped into the matrix. Rows >= M compute garbage
+      // that is never used: the last GEMM does not store them, and an intermediate's
+      // padded rows meet zero A columns in the next GEMM (INT8 zeroes them anyway,
+      // because they enter its column sums).
+      const char *rowp[RP];
+      bool row_ok[RP];
+#pragma unroll
+      for (int i = 0; i < RP; ++i) {
+        const long row = rb0 + i * 32 + l32;
+        row_ok[i] = row < M;
+        rowp[i] = A + ((size_t)(row_ok[i] ? row : M - 1) * K + h * T::KA) * T::ESZ;
+      }
       AFrag<KIND> fa[2][PF][RP];
       auto load_chunk = [&](int c, AFrag<KIND>(&dst)[PF][RP]) {
 #pragma unroll
         for (int s = 0; s < PF; ++s) {
-          long k = (long)(c * PF + s) * T::KC + h * T::KA;
+          const size_t off = (size_t)(c * PF + s) * T::KC * T::ESZ;
 #pragma unroll
-          for (int i = 0; i < RP; ++i) dst[s][i] = load_afrag<KIND>(A, M, K, rb0 + i * 32 + l32, k, vec);
+          for (int i = 0; i < RP; ++i) {
+#pragma unroll
+            for (int q = 0; q < (KIND == K_F32X3 ? 2 : 1); ++q) {
+              uint4 v = *(const uint4 *)(rowp[i] + off + 16 * q);
+              if (KIND == K_I8) {  // u8 -> s8 bias
+                v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
+              }
+              dst[s][i].v[q] = v;
+            }
+          }
         }
       };
+      // B fragments one step ahead, double buffered (PF is even, so the buffer of
+      // step ks is ks & 1 across chunks). The read for the step after the last
+      // clamps to the last step: harmless and branch free.
+      BFrag<KIND, TNB> bq[2];
+      const int glast = xg0 + 2 * (ksteps - 1) + h;
       auto run_chunk = [&](int c, const AFrag<KIND>(&src)[PF][RP]) {
 #pragma unroll
         for (int s = 0; s < PF; ++s) {
-          int ks = c * PF + s;
-          if (ks < ksteps) mma_k<KIND, TNB>(arena, lo_off, xg0 + 2 * ks + h, lane, src[s], acc);
+          const int ks = c * PF + s;
+          const int gn = xg0 + 2 * (ks + 1) + h;
+          read_bfrag<KIND, TNB>(arena, lo_off, gn < glast ? gn : glast, l32, bq[(s + 1) & 1]);
+          mma_k<KIND, TNB>(bq[s & 1], src[s], acc);
         }
       };
-      load_chunk(0, fa[0]);
-      for (int c = 0; c < nchunks; c += 2) {
-        if (c + 1 < nchunks) load_chunk(c + 1, fa[1]);
+      read_bfrag<KIND, TNB>(arena, lo_off, xg0 + h, l32, bq[0]);
+      if (nfc > 0) load_chunk(0, fa[0]);
+      for (int c = 0; c < nfc; c += 2) {
+        if (c + 1 < nfc) load_chunk(c + 1, fa[1]);
         run_chunk(c, fa[0]);
-        if (c + 1 < nchunks) {
-          if (c + 2 < nchunks) load_chunk(c + 2, fa[0]);
+        if (c + 1 < nfc) {
+          if (c + 2 < nfc) load_chunk(c + 2, fa[0]);
           run_chunk(c + 1, fa[1]);
         }
+      }
+      // K tail: checked loads, one step at a time
+      for (int ks = nfc * PF; ks < ksteps; ++ks) {
+        AFrag<KIND> af[RP];
+        const long k = (long)ks * T::KC + h * T::KA;
+#pragma unroll
+        for (int i = 0; i < RP; ++i) af[i] = load_afrag<KIND>(rowp[i] - h * T::KA * T::ESZ, K, k, row_ok[i]);
+        BFrag<KIND, TNB> bt;
+        read_bfrag<KIND, TNB>(arena, lo_off, xg0 + 2 * ks + h, l32, bt);
+        mma_k<KIND, TNB>(bt, af, acc);
       }
 
       // epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32
 #pragma unroll
       for (int j = 0; j < TNB; ++j) {
         const int n = j * 32 + l32;
-        int bias = 0;
-        if (KIND == K_I8) bias = 128 * colsum[n];
 #pragma unroll
         for (int i = 0; i < RP; ++i) {
           const long rbase = rb0 + i * 32;
@@ -331,7 +365,7 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
               long m = rbase + 8 * (r >> 2) + 4 * h + (r & 3);
               if (m >= M) continue;
               size_t off = (size_t)m * P.N + n0 + n;
-              if (KIND == K_I8) ((int *)P.C)[off] = (int)((unsigned)acc[i][j][r] + (unsigned)bias);
+              if (KIND == K_I8) ((int *)P.C)[off] = (int)acc[i][j][r];
               else ((float *)P.C)[off] = (float)acc[i][j][r];
             }
           } else {
@@ -347,24 +381,22 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
                                   __builtin_bit_cast(uint32_t, (float)acc[i][j][4 * q + 2]),
                                   __builtin_bit_cast(uint32_t, (float)acc[i][j][4 * q + 3]));
               } else if (KIND == K_BF16) {
-                uint2 w = make_uint2(pk(bf16_rne(acc[i][j][4 * q]), bf16_rne(acc[i][j][4 * q + 1])),
-                                     pk(bf16_rne(acc[i][j][4 * q + 2]), bf16_rne(acc[i][j][4 * q + 3])));
-                *((uint2 *)dst + (pos >> 2)) = w;
+                *((uint2 *)dst + (pos >> 2)) =
+                    make_uint2(cvt_pk_bf16(acc[i][j][4 * q], acc[i][j][4 * q + 1]),
+                               cvt_pk_bf16(acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]));
               } else if (KIND == K_I8) {
                 // padded rows (m >= M) must stay zero: they enter the next column sum
-                uint32_t w = 0u;
                 const long m0r = rbase + 8 * q + 4 * h;
-#pragma unroll
-                for (int e = 0; e < 4; ++e)
-                  if (m0r + e < M)
-                    w |= (uint32_t)(uint8_t)sat_i8((int)((unsigned)acc[i][j][4 * q + e] + (unsigned)bias)) << (8 * e);
+                uint32_t w = sat_pk_i8((int)acc[i][j][4 * q], (int)acc[i][j][4 * q + 1], (int)acc[i][j][4 * q + 2],
+                                       (int)acc[i][j][4 * q + 3]);
+                if (m0r + 4 > M) w &= m0r >= M ? 0u : (0xffffffffu >> (8 * (m0r + 4 - M)));
                 *((uint32_t *)dst + (pos >> 2)) = w;
               } else {
-                uint16_t hh[4], ll[4];
-#pragma unroll
-                for (int e = 0; e < 4; ++e) split_bf16((float)acc[i][j][4 * q + e], hh[e], ll[e]);
-                *((uint2 *)dst + (pos >> 2)) = make_uint2(pk(hh[0], hh[1]), pk(hh[2], hh[3]));
-                *((uint2 *)(arena + lo_off + cslot<TN>(g, n)) + (pos >> 2)) = make_uint2(pk(ll[0], ll[1]), pk(ll[2], ll[3]));
+                uint32_t h01, l01, h23, l23;
+                split_pk(acc[i][j][4 * q], acc[i][j][4 * q + 1], h01, l01);
+                split_pk(acc[i][j][4 * q + 2], acc[i][j][4 * q + 3], h23, l23);
+                *((uint2 *)dst + (pos >> 2)) = make_uint2(h01, h23);
+                *((uint2 *)(arena + lo_off + cslot<TN>(g, n)) + (pos >> 2)) = make_uint2(l01, l23);
               }
             }
           }
@@ -410,6 +442,7 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   using T = CT<KIND>;
   ChainParams P{};
   make_chain_params(c, &P);
+  if (!chain_fused_aligned(P)) return hipErrorInvalidValue;
   constexpr int TN = TNB * 32;
   size_t lds = (size_t)P.S * TN * T::LB + (KIND == K_I8 ? TN * sizeof(int) : 0);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
@@ -423,6 +456,13 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
 
 // Tile widths of the grouped kernel, one per kind.
 constexpr int GT_F32 = 1, GT_BF16 = 1, GT_I8 = 2, GT_F32X3 = 1;
+
+// Out-of-line tile bodies for the grouped kernel: each kind keeps its own register
+// allocation instead of one allocation for the union of all four.
+template <int KIND, int TNB>
+__device__ __noinline__ void chain_tile_call(const ChainParams &P, long n0, uint4 *arena) {
+  chain_tile<KIND, TNB>(P, n0, arena);
+}
 
 // Persistent grouped launch over a packed table (GroupTable header, the request
 // descriptors, the tile list): workgroup b runs tiles b, b + grid, ...; each tile is
@@ -438,10 +478,10 @@ __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restri
     const int2 w = items[it];
     const ChainParams &P = reqs[w.x];
     switch (P.kind) {
-      case K_F32: chain_tile<K_F32, GT_F32>(P, (long)w.y * GT_F32 * 32, arena); break;
-      case K_BF16: chain_tile<K_BF16, GT_BF16>(P, (long)w.y * GT_BF16 * 32, arena); break;
-      case K_I8: chain_tile<K_I8, GT_I8>(P, (long)w.y * GT_I8 * 32, arena); break;
-      case K_F32X3: chain_tile<K_F32X3, GT_F32X3>(P, (long)w.y * GT_F32X3 * 32, arena); break;
+      case K_F32: chain_tile_call<K_F32, GT_F32>(P, (long)w.y * GT_F32 * 32, arena); break;
+      case K_BF16: chain_tile_call<K_BF16, GT_BF16>(P, (long)w.y * GT_BF16 * 32, arena); break;
+      case K_I8: chain_tile_call<K_I8, GT_I8>(P, (long)w.y * GT_I8 * 32, arena); break;
+      case K_F32X3: chain_tile_call<K_F32X3, GT_F32X3>(P, (long)w.y * GT_F32X3 * 32, arena); break;
     }
     __syncthreads();  // the next tile restages the arena
   }
@@ -486,6 +526,21 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
   P->C = c.C;
   P->N = c.N;
   P->vecB = (c.N % 4 == 0) && (((uintptr_t)c.B & (4 * e - 1)) == 0);
+}
+
+bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K) {
+  static const int esz[4] = {4, 2, 1, 4};
+  if (num_gemm < 1 || num_gemm > MAXG || N % 4 != 0) return false;
+  for (int i = 0; i < num_gemm; ++i)
+    if ((K[i] * esz[kind]) % 16 != 0) return false;
+  return true;
+}
+
+bool chain_fused_aligned(const ChainParams &P) {
+  if (!P.vecB) return false;
+  for (int i = 0; i < P.num_gemm; ++i)
+    if (!P.vecA[i]) return false;
+  return true;
 }
 
 int grouped_tile_n(int kind) {

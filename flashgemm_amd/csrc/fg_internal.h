@@ -52,14 +52,46 @@ __host__ __device__ inline uint16_t bf16_rne(float x) {
 __host__ __device__ inline float bf16_to_f32(uint16_t b) {
   return __builtin_bit_cast(float, (uint32_t)b << 16);
 }
-// hi/lo split used by the compensated FP32 form: hi = rne(x), lo = rne(x - hi).
-// x - hi is exact in fp32 (Sterbenz), so hi + lo carries 16 significand bits.
-__device__ inline void split_bf16(float x, uint16_t &hi, uint16_t &lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - bf16_to_f32(hi));
-}
 
 __device__ inline int8_t sat_i8(int v) { return (int8_t)(v > 127 ? 127 : (v < -128 ? -128 : v)); }
+
+// Two fp32 -> two bf16 (low half = first), round to nearest even, one gfx950
+// v_cvt_pk_bf16_f32. Unlike bf16_rne above, a denormal fp32 input (|x| < 2^-126)
+// rounds to a bf16 denormal instead of flushing to zero.
+typedef __attribute__((ext_vector_type(2))) float f32x2_t;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
+__device__ __forceinline__ uint32_t cvt_pk_bf16(float a, float b) {
+  f32x2_t v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2_t));
+}
+// Device-side fp32 -> bf16 of every kernel here (one rounding everywhere, so the
+// fused and unfused paths produce identical intermediates).
+__device__ __forceinline__ uint16_t bf16_dev(float x) { return (uint16_t)cvt_pk_bf16(x, 0.0f); }
+// hi/lo split of two floats, packed: hi = rne(x), lo = rne(x - hi); x - hi is exact
+// in fp32 (Sterbenz), so hi + lo carries 16 significand bits.
+__device__ __forceinline__ void split_pk(float a, float b, uint32_t &hi, uint32_t &lo) {
+  hi = cvt_pk_bf16(a, b);
+  const float ha = __builtin_bit_cast(float, hi << 16), hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+  lo = cvt_pk_bf16(a - ha, b - hb);
+}
+__device__ __forceinline__ void split_bf16(float x, uint16_t &hi, uint16_t &lo) {
+  uint32_t h, l;
+  split_pk(x, 0.0f, h, l);
+  hi = (uint16_t)h;
+  lo = (uint16_t)l;
+}
+// Four int32 saturated to int8 and packed little-endian into one dword: clamps
+// become v_med3_i32, the packing two v_perm_b32 (selector byte 0x0c yields zero;
+// selectors 0-3 pick bytes of the second operand, 4-7 of the first).
+__device__ __forceinline__ uint32_t sat_pk_i8(int a, int b, int c, int d) {
+  a = a < -128 ? -128 : (a > 127 ? 127 : a);
+  b = b < -128 ? -128 : (b > 127 ? 127 : b);
+  c = c < -128 ? -128 : (c > 127 ? 127 : c);
+  d = d < -128 ? -128 : (d > 127 ? 127 : d);
+  const uint32_t ab = __builtin_amdgcn_perm((uint32_t)b, (uint32_t)a, 0x0c0c0400u);
+  const uint32_t cd = __builtin_amdgcn_perm((uint32_t)d, (uint32_t)c, 0x04000c0cu);
+  return ab | cd;
+}
 
 // ------------------------------------------------------ reference layout --
 // The reference's GEMM i epilogue stores the requantized panel so that GEMM i+1
@@ -149,6 +181,10 @@ struct ChainParams {
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);
+// The fused kernels need every A row and the B tile in whole 16-byte / 4-element
+// groups: K_i * elem % 16 == 0 and N % 4 == 0 (shape), and aligned base pointers.
+bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K);
+bool chain_fused_aligned(const ChainParams &P);
 
 // Fused chain: every intermediate stays in LDS. Returns hipErrorInvalidValue when the
 // largest (K_i + M_i) x tile_n panel does not fit the LDS budget.

@@ -65,6 +65,17 @@ struct Stager {
   // A tile rows [m0, m0+128) x k [k0, k0+BK): 1024 16-byte units, 4 per thread.
   __device__ __forceinline__ void load_a(const char *A, long M, long K, long kend, long m0, long k0,
                                          bool vec, int t) {
+    if (vec && m0 + 128 <= M && k0 + T::BK <= kend) {  // interior tile (uniform branch)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        int u = t + NT * q;
+        const uint4 val = *(const uint4 *)(A + ((size_t)(m0 + (u >> 3)) * K + k0 + (long)(u & 7) * (16 / T::ESZ)) *
+                                                   T::ESZ);
+        const uint32_t x = KIND == K_I8 ? 0x80808080u : 0u;  // u8 -> s8 bias
+        a[4 * q + 0] = val.x ^ x; a[4 * q + 1] = val.y ^ x; a[4 * q + 2] = val.z ^ x; a[4 * q + 3] = val.w ^ x;
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       int u = t + NT * q;
@@ -109,13 +120,14 @@ struct Stager {
       int row = u >> 3, v = u & 7;
       if (KIND == K_F32X3) {
         // 4 floats = k 4v..4v+3 -> group v/2, half v%2 of the hi and lo planes.
-        uint16_t h[4], l[4];
-        for (int e = 0; e < 4; ++e) split_bf16(__builtin_bit_cast(float, a[4 * q + e]), h[e], l[e]);
+        uint32_t h01, l01, h23, l23;
+        split_pk(__builtin_bit_cast(float, a[4 * q]), __builtin_bit_cast(float, a[4 * q + 1]), h01, l01);
+        split_pk(__builtin_bit_cast(float, a[4 * q + 2]), __builtin_bit_cast(float, a[4 * q + 3]), h23, l23);
         int g = v >> 1, half = v & 1;
         uint2 *hp = (uint2 *)(sA + slot(g, row)) + half;
         uint2 *lp = (uint2 *)(sA + TILE_U4 / 2 + slot(g, row)) + half;
-        *hp = make_uint2(pack_bf16x2(h[0], h[1]), pack_bf16x2(h[2], h[3]));
-        *lp = make_uint2(pack_bf16x2(l[0], l[1]), pack_bf16x2(l[2], l[3]));
+        *hp = make_uint2(h01, h23);
+        *lp = make_uint2(l01, l23);
       } else {
         sA[slot(v, row)] = make_uint4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
       }
@@ -130,6 +142,22 @@ struct Stager {
     long n = n0 + (long)cb * T::CW;
     long kb = k0 + (long)kg * T::RB;
     constexpr int RBYTES = T::CW * T::ESZ;  // bytes per row piece: 16, 8, 4 or 8
+    if (vec && n0 + 128 <= N && k0 + T::BK <= kend) {  // interior tile (uniform branch)
+#pragma unroll
+      for (int r = 0; r < T::RB; ++r) {
+        const char *p = B + ((size_t)(kb + r) * N + n) * T::ESZ;
+        if (RBYTES == 16) {
+          uint4 v = *(const uint4 *)p;
+          b[4 * r + 0] = v.x; b[4 * r + 1] = v.y; b[4 * r + 2] = v.z; b[4 * r + 3] = v.w;
+        } else if (RBYTES == 8) {
+          uint2 v = *(const uint2 *)p;
+          b[2 * r + 0] = v.x; b[2 * r + 1] = v.y;
+        } else {
+          b[r] = *(const uint32_t *)p;
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int r = 0; r < T::RB; ++r) {
       long k = kb + r;
@@ -191,12 +219,11 @@ struct Stager {
         }
         sB[slot(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
       } else {  // F32X3: 8 floats of column c -> hi and lo planes
-        uint16_t h[8], l[8];
-        for (int r = 0; r < 8; ++r) split_bf16(__builtin_bit_cast(float, b[2 * r + c]), h[r], l[r]);
-        sB[slot(g, n)] = make_uint4(pack_bf16x2(h[0], h[1]), pack_bf16x2(h[2], h[3]),
-                                    pack_bf16x2(h[4], h[5]), pack_bf16x2(h[6], h[7]));
-        sB[TILE_U4 / 2 + slot(g, n)] = make_uint4(pack_bf16x2(l[0], l[1]), pack_bf16x2(l[2], l[3]),
-                                                  pack_bf16x2(l[4], l[5]), pack_bf16x2(l[6], l[7]));
+        uint32_t h[4], l[4];
+        for (int r = 0; r < 4; ++r)
+          split_pk(__builtin_bit_cast(float, b[4 * r + c]), __builtin_bit_cast(float, b[4 * r + 2 + c]), h[r], l[r]);
+        sB[slot(g, n)] = make_uint4(h[0], h[1], h[2], h[3]);
+        sB[TILE_U4 / 2 + slot(g, n)] = make_uint4(l[0], l[1], l[2], l[3]);
       }
     }
   }
@@ -348,7 +375,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, c
           if (EPI == EPI_STORE) ((int *)Cv)[off] = v;
           else ((int8_t *)Cv)[off] = sat_i8(v);
         } else if (KIND == K_BF16 && EPI == EPI_REQUANT) {
-          ((uint16_t *)Cv)[off] = bf16_rne((float)acc[i][j][r]);
+          ((uint16_t *)Cv)[off] = bf16_dev((float)acc[i][j][r]);
         } else {
           ((float *)Cv)[off] = (float)acc[i][j][r];
         }
@@ -356,13 +383,38 @@ __global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, c
   }
 }
 
-// INT8 column sums sum_k s8(B[k][n]) (exact in int32 for K < 2^24).
-__global__ void colsum_i8_kernel(const int8_t *__restrict__ B, long K, long N, int *__restrict__ out) {
+// INT8 column sums sum_k s8(B[k][n]) (exact in int32 for K < 2^24). A thread sums
+// CS_ROWS rows of 4 adjacent columns (one dword per row, coalesced across the
+// block), then adds its partial into out[] with integer atomics: integer addition
+// is exact and order independent, so the result is deterministic. out[] must be
+// zero first (colsum_zero_kernel, a fill kernel rather than a memset so the pair
+// can sit inside a captured graph).
+constexpr int CS_ROWS = 64;
+
+__global__ void colsum_zero_kernel(int *__restrict__ out, long N) {
   long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < N) out[n] = 0;
+}
+
+__global__ void colsum_i8_kernel(const int8_t *__restrict__ B, long K, long N, int *__restrict__ out) {
+  const long n = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  const long k0 = (long)blockIdx.y * CS_ROWS;
   if (n >= N) return;
-  int s = 0;
-  for (long k = 0; k < K; ++k) s += B[(size_t)k * N + n];
-  out[n] = s;
+  const long k1 = (k0 + CS_ROWS < K) ? k0 + CS_ROWS : K;
+  int s[4] = {0, 0, 0, 0};
+  if (n + 4 <= N && (N & 3) == 0) {
+    for (long k = k0; k < k1; ++k) {
+      const uint32_t w = *(const uint32_t *)(B + (size_t)k * N + n);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) s[c] += (int)(int8_t)(w >> (8 * c));
+    }
+  } else {
+    for (long k = k0; k < k1; ++k)
+      for (int c = 0; c < 4; ++c)
+        if (n + c < N) s[c] += B[(size_t)k * N + n + c];
+  }
+  for (int c = 0; c < 4; ++c)
+    if (n + c < N) atomicAdd(&out[n + c], s[c]);
 }
 
 // Fixed-order split-K reduction: partial slabs are added in slab order 0..S-1, so
@@ -382,7 +434,7 @@ __global__ void splitk_reduce_kernel(const void *__restrict__ P, int splits, lon
   } else {
     float s = 0.f;
     for (int z = 0; z < splits; ++z) s += ((const float *)P)[z * MN + e];
-    if (KIND == K_BF16 && EPI == EPI_REQUANT) ((uint16_t *)C)[e] = bf16_rne(s);
+    if (KIND == K_BF16 && EPI == EPI_REQUANT) ((uint16_t *)C)[e] = bf16_dev(s);
     else ((float *)C)[e] = s;
   }
 }
@@ -440,7 +492,9 @@ hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
 }
 
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s) {
-  hipLaunchKernelGGL(colsum_i8_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, B, K, N, out);
+  hipLaunchKernelGGL(colsum_zero_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, out, N);
+  dim3 grid((unsigned)((N + 1023) / 1024), (unsigned)((K + CS_ROWS - 1) / CS_ROWS));
+  hipLaunchKernelGGL(colsum_i8_kernel, grid, dim3(256), 0, s, B, K, N, out);
   return hipGetLastError();
 }
 

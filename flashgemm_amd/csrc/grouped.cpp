@@ -55,8 +55,8 @@ int describe(const fg_chain_req &r, ChainParams *P, int *tile_n, size_t *lds, bo
   *tile_n = grouped_tile_n(kind);
   c.tile_n = *tile_n;
   *lds = chain_fused_lds_bytes(kind, r.M, r.K, r.num_gemm, *tile_n);
-  *fits = *lds <= LDS_BYTES_MAX;
   make_chain_params(c, P);
+  *fits = *lds <= LDS_BYTES_MAX && chain_fused_shape_ok(kind, r.N, r.num_gemm, r.K) && chain_fused_aligned(*P);
   return FG_OK;
 }
 

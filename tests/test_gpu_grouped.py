@@ -12,16 +12,20 @@ from test_gpu_parity import TOL_BF16, TOL_F32, dev, empty, rand_chain
 pytestmark = pytest.mark.gpu
 
 
-def make_requests(rng, count):
-    """Mixed dtypes, chain lengths 1..4, uneven N, M and K (multiples of 32 and not)."""
+def make_requests(rng, count, fitting=False):
+    """Mixed dtypes, chain lengths 1..4, uneven N, M and K (multiples of 32 and not).
+    fitting=True keeps every request on the grouped on-chip path (N % 4 == 0 and
+    widths in whole 16-byte groups for every dtype)."""
     import torch
     reqs = []
-    widths = [32, 48, 64, 96, 128, 200, 256]
+    widths = [32, 48, 64, 96, 128, 256] if fitting else [32, 48, 64, 96, 128, 200, 256]
     for r in range(count):
         dtype = [fg.DTYPE_BF16, fg.DTYPE_INT8, fg.DTYPE_F32][r % 3]
         n = 1 + r % 4
         dims = [int(rng.choice(widths)) for _ in range(n + 1)]
         N = int(rng.integers(1, 300))
+        if fitting:
+            N = 4 * (N // 4 + 1)
         B, As = rand_chain(rng, dtype, dims[0], N, dims[1:])
         C = empty((dims[-1], N), torch.int32 if dtype == fg.DTYPE_INT8 else torch.float32)
         dB, dA = dev(B), [dev(a) for a in As]
@@ -41,10 +45,13 @@ def check_against_oracle(q, got):
 
 
 def single_launch(q):
+    """The same chain alone through fg_chain with default options: the planner picks
+    the fused path exactly when the request fits a grouped tile (same k order), and
+    otherwise the same unfused plan the grouped launch falls back to."""
     import torch
     C = torch.empty_like(q["C"])
     fg.chain(q["dtype"], C, q["dB"], q["dA"], N=q["N"], Ms=[a.shape[0] for a in q["As"]],
-             Ks=[a.shape[1] for a in q["As"]], o=fg.opts(fuse=fg.FUSE_ON, tile_n=64 if q["dtype"] == fg.DTYPE_INT8 else 32))
+             Ks=[a.shape[1] for a in q["As"]])
     torch.cuda.synchronize()
     return C.cpu().numpy()
 
@@ -82,7 +89,8 @@ def test_graph_replays_match_eager(gpu):
     g = fg.GroupedGraph(max_reqs=64, max_tiles=4096)
     try:
         for rep, count in enumerate([5, 17, 3, 17]):
-            reqs = make_requests(rng, count)
+            reqs = make_requests(rng, count, fitting=True)
+            assert all(fg.grouped_fits(q["req"]) for q in reqs)
             print(f"replay {rep}: {count} requests", flush=True)
             g.launch([q["req"] for q in reqs])
             torch.cuda.synchronize()

@@ -162,6 +162,19 @@ def test_chain_golden_through_reference_entry_point(gpu, name):
         assert O.rel_fro(C, d["C"]) <= TOL_F32
 
 
+def expect_unfusable(dtype, K1, N, Ms, fuse):
+    """When the fused path cannot take a shape (arena of (K_i + M_i) rows beyond LDS,
+    N % 4 != 0, or a K_i row not a whole number of 16-byte groups), asking for it
+    must fail with FG_ERR_UNSUPPORTED rather than run something else. Returns True
+    when that was checked (nothing more to test for this case)."""
+    if fuse != fg.FUSE_ON or fg.chain_plan(dtype, N, Ms, [K1] + Ms[:-1])[0]:
+        return False
+    with pytest.raises(fg.FlashgemmError) as e:
+        run_chain(dtype, *rand_chain(np.random.default_rng(0), dtype, K1, N, Ms), fuse=fuse)
+    assert e.value.status == -7
+    return True
+
+
 CHAIN_SHAPES = [
     (64, 64, [32]),
     (64, 200, [96, 64]),                 # ragged N
@@ -175,6 +188,8 @@ CHAIN_SHAPES = [
 @pytest.mark.parametrize("fuse", [fg.FUSE_ON, fg.FUSE_OFF])
 def test_int8_chain_bit_exact_math_layout(gpu, K1, N, Ms, fuse):
     print(f"int8 chain K1={K1} N={N} Ms={Ms} fuse={fuse}", flush=True)
+    if expect_unfusable(fg.DTYPE_INT8, K1, N, Ms, fuse):
+        return
     rng = np.random.default_rng(K1 + N)
     B, As = rand_chain(rng, fg.DTYPE_INT8, K1, N, Ms)
     got = run_chain(fg.DTYPE_INT8, B, As, fuse=fuse)
@@ -194,6 +209,8 @@ def test_int8_chain_every_tile_width(gpu, tile_n):
 @pytest.mark.parametrize("fuse", [fg.FUSE_ON, fg.FUSE_OFF])
 def test_bf16_chain_math_layout(gpu, K1, N, Ms, fuse):
     print(f"bf16 chain K1={K1} N={N} Ms={Ms} fuse={fuse}", flush=True)
+    if expect_unfusable(fg.DTYPE_BF16, K1, N, Ms, fuse):
+        return
     rng = np.random.default_rng(K1 * 3 + N)
     B, As = rand_chain(rng, fg.DTYPE_BF16, K1, N, Ms)
     got = run_chain(fg.DTYPE_BF16, B, As, fuse=fuse)
@@ -203,11 +220,7 @@ def test_bf16_chain_math_layout(gpu, K1, N, Ms, fuse):
 @pytest.mark.parametrize("K1,N,Ms", CHAIN_SHAPES)
 @pytest.mark.parametrize("fuse", [fg.FUSE_ON, fg.FUSE_OFF])
 def test_f32_chain(gpu, K1, N, Ms, fuse):
-    Ks = [K1] + Ms[:-1]
-    if fuse == fg.FUSE_ON and not fg.chain_plan(fg.DTYPE_F32, N, Ms, Ks)[0]:
-        with pytest.raises(fg.FlashgemmError) as e:  # fp32 arena of (K_i + M_i) rows exceeds LDS
-            run_chain(fg.DTYPE_F32, *rand_chain(np.random.default_rng(0), fg.DTYPE_F32, K1, N, Ms), fuse=fuse)
-        assert e.value.status == -7
+    if expect_unfusable(fg.DTYPE_F32, K1, N, Ms, fuse):
         return
     rng = np.random.default_rng(K1 * 5 + N)
     B, As = rand_chain(rng, fg.DTYPE_F32, K1, N, Ms)

@@ -40,13 +40,15 @@ class FlashgemmError(RuntimeError):
 
 class ChainOpts(ctypes.Structure):
     _fields_ = [("layout", ctypes.c_int), ("ref_threads", ctypes.c_int), ("f32_mode", ctypes.c_int),
-                ("fuse", ctypes.c_int), ("tile_n", ctypes.c_int), ("splits", ctypes.c_int)]
+                ("fuse", ctypes.c_int), ("tile_n", ctypes.c_int), ("splits", ctypes.c_int),
+                ("a_packed", ctypes.c_int)]
 
 
 class ChainReq(ctypes.Structure):
     _fields_ = [("dtype", ctypes.c_int), ("f32_mode", ctypes.c_int), ("C", ctypes.c_void_p),
                 ("B", ctypes.c_void_p), ("N", ctypes.c_long), ("num_gemm", ctypes.c_int),
-                ("A", ctypes.c_void_p * 8), ("M", ctypes.c_long * 8), ("K", ctypes.c_long * 8)]
+                ("A", ctypes.c_void_p * 8), ("M", ctypes.c_long * 8), ("K", ctypes.c_long * 8),
+                ("a_packed", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -84,6 +86,9 @@ def lib() -> ctypes.CDLL:
         L.LibShalom_set_thread_nums.argtypes = [i]
         L.LibShalom_sgemm_mp.argtypes = [i, i, vp, vp, vp, l, l, l]
         L.flashgemm_sgemm.argtypes = [i, i, vp, vp, vp, l, l, l]
+        L.fg_packed_weight_bytes.argtypes = [i, i, l, l]
+        L.fg_packed_weight_bytes.restype = ctypes.c_size_t
+        L.fg_pack_weight.argtypes = [i, i, vp, l, l, vp, vp]
         L.fg_grouped_chain.argtypes = [i, ctypes.POINTER(ChainReq), vp]
         L.fg_grouped_fits.argtypes = [ctypes.POINTER(ChainReq)]
         L.fg_grouped_graph_create.argtypes = [i, i, ctypes.POINTER(vp)]
@@ -117,8 +122,9 @@ def last_error() -> str:
     return lib().fg_last_error().decode(errors="replace")
 
 
-def opts(layout=LAYOUT_MATH, ref_threads=1, f32_mode=F32_MFMA, fuse=FUSE_AUTO, tile_n=0, splits=0) -> ChainOpts:
-    return ChainOpts(layout, ref_threads, f32_mode, fuse, tile_n, splits)
+def opts(layout=LAYOUT_MATH, ref_threads=1, f32_mode=F32_MFMA, fuse=FUSE_AUTO, tile_n=0, splits=0,
+         a_packed=0) -> ChainOpts:
+    return ChainOpts(layout, ref_threads, f32_mode, fuse, tile_n, splits, a_packed)
 
 
 def _addr(x) -> int:
@@ -199,6 +205,16 @@ def error_norms(x_kind: int, x, ref, n: int, stream=None):
     return out[0], out[1]
 
 
+def packed_weight_bytes(dtype: int, M: int, K: int, f32_mode: int = F32_MFMA) -> int:
+    """Bytes of one weight matrix in the fused kernels' packed layout."""
+    return lib().fg_packed_weight_bytes(dtype, f32_mode, M, K)
+
+
+def pack_weight(dtype: int, A, M: int, K: int, packed, f32_mode: int = F32_MFMA, stream=None) -> None:
+    """Pack A (M x K, device) into `packed` for chains run with a_packed=1."""
+    _check(lib().fg_pack_weight(dtype, f32_mode, _addr(A), M, K, _addr(packed), _stream(stream)))
+
+
 def synchronize(stream=None) -> None:
     _check(lib().fg_synchronize(_stream(stream)))
 
@@ -241,9 +257,10 @@ def LibShalom_sgemm_mp(transa: int, transb: int, C, A, B, M: int, N: int, K: int
 
 # ------------------------------------------------------ grouped launches --
 
-def chain_req(dtype: int, C, B, As, N: int | None = None, f32_mode: int = F32_MFMA, Ms=None, Ks=None) -> ChainReq:
+def chain_req(dtype: int, C, B, As, N: int | None = None, f32_mode: int = F32_MFMA, Ms=None, Ks=None,
+              a_packed: int = 0) -> ChainReq:
     r = ChainReq()
-    r.dtype, r.f32_mode = dtype, f32_mode
+    r.dtype, r.f32_mode, r.a_packed = dtype, f32_mode, a_packed
     r.C, r.B = _addr(C), _addr(B)
     r.N = N if N is not None else B.shape[1]
     r.num_gemm = len(As)

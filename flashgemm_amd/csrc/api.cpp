@@ -133,6 +133,26 @@ void fg_chain_opts_default(fg_chain_opts *o) {
   o->fuse = FG_FUSE_AUTO;
   o->tile_n = 0;
   o->splits = 0;
+  o->a_packed = 0;
+}
+
+size_t fg_packed_weight_bytes(int dtype, int f32_mode, long M, long K) {
+  if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8 || M <= 0 || K <= 0) return 0;
+  return packed_a_bytes(kind_of(dtype, f32_mode), M, K);
+}
+
+int fg_pack_weight(int dtype, int f32_mode, const void *A, long M, long K, void *packed, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8 || !A || !packed || M <= 0 || K <= 0)
+    return set_error(FG_ERR_INVALID, "pack_weight: dtype=%d M=%ld K=%ld", dtype, M, K);
+  PackJob J{};
+  J.n = 1;
+  J.A[0] = A;
+  J.M[0] = M;
+  J.K[0] = K;
+  J.out[0] = packed;
+  return hip_status(launch_pack_a(kind_of(dtype, f32_mode), J, (hipStream_t)stream), "pack weight");
 }
 
 const char *fg_status_string(int s) {
@@ -308,11 +328,34 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
     c.tile_n = tn;
     ChainParams P;
     make_chain_params(c, &P);
-    if (chain_fused_aligned(P)) return hip_status(launch_chain_fused(c, s), "fused chain");
-    if (o.fuse == FG_FUSE_ON)
-      return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but A or B is not 16-byte aligned");
-    // misaligned base pointers: run unfused
+    if (chain_fused_aligned(P)) {
+      if (!o.a_packed) {
+        // pack every A_i into MFMA fragment order (per-stream scratch, one launch)
+        size_t off[CHAIN_MAXG], total = 0;
+        for (int i = 0; i < num_gemm; ++i) {
+          off[i] = total;
+          total += (packed_a_bytes(kind, M[i], K[i]) + 255) & ~(size_t)255;
+        }
+        char *pk = nullptr;
+        if ((rc = scratch(s, "apack", total, (void **)&pk))) return rc;
+        PackJob J{};
+        J.n = num_gemm;
+        for (int i = 0; i < num_gemm; ++i) {
+          J.A[i] = A[i];
+          J.M[i] = M[i];
+          J.K[i] = K[i];
+          J.out[i] = pk + off[i];
+          c.A[i] = pk + off[i];
+        }
+        if ((rc = hip_status(launch_pack_a(kind, J, s), "pack A"))) return rc;
+      }
+      return hip_status(launch_chain_fused(c, s), "fused chain");
+    }
+    if (o.fuse == FG_FUSE_ON || o.a_packed)
+      return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but B is not aligned to 4 elements");
+    // misaligned B: run unfused
   }
+  if (o.a_packed) return set_error(FG_ERR_UNSUPPORTED, "packed weights need the fused path");
 
   // Unfused: GEMM i writes its requantized output to HBM; the reference layout adds
   // a gather that re-reads it the way the reference's next GEMM does.

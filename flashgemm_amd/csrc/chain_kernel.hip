@@ -5,15 +5,21 @@
 // B and runs every GEMM of the chain, the epilogue of GEMM i writing straight into
 // GEMM i+1's packed B (flashgemm.c:132). Here a workgroup owns a tile of `TN`
 // columns of B and runs every GEMM of the chain on it:
+//   * the weights are first re-laid out by pack_a_kernel into MFMA fragment order
+//     (the counterpart of the reference's per-call A pack, PACK_x86.h:616-647):
+//     one wave's 16-byte-per-lane fragment of 32 rows is 1 KB contiguous, so the
+//     hot loop's weight loads are fully coalesced, rows and K are zero padded (no
+//     bounds logic), INT8 bytes carry the u8 -> s8 bias (x ^ 0x80) and FP32 for the
+//     split-bf16 form is stored as its bf16 hi/lo halves;
 //   * the B tile (K1 x TN) is staged once into an LDS arena in the MFMA B-operand
 //     layout (16-byte k-groups per column, slot g*TN + (n ^ (g & 7)));
 //   * GEMM i reads X_i from one end of the arena and writes its requantized output
 //     Y_i (fp32 / bf16 RNE / int8 saturate) at the other end, so X_{i+1} = Y_i never
 //     leaves the CU; the arena holds max_i (K_i + M_i) rows;
-//   * the weights A_i stream from L2 straight into registers (each row block is
-//     read by exactly one wave), double buffered PF k-steps ahead;
-//   * INT8 uses u8(a).b = s8(a ^ 0x80).b + 128 sum_k b, the column sums of each
-//     X_i taken from LDS before its GEMM;
+//   * each wave owns row blocks of the weights, streams their packed fragments PF
+//     k-steps ahead (double buffered) and reads B fragments one step ahead;
+//   * INT8 uses u8(a).b = s8(a ^ 0x80).b + 128 sum_k b: the column sums of each X_i
+//     are taken from LDS and seed the accumulators;
 //   * only the last GEMM writes HBM (C, fp32 or int32, row-major).
 #include "fg_internal.h"
 
@@ -27,60 +33,100 @@ constexpr size_t LDS_LIMIT = 160 * 1024;
 
 template <int KIND> struct CT;
 // KG: k per 16-byte LDS group (per plane for F32X3); KC: k per MFMA step;
-// KA: A elements one lane loads per step; ESZ: global element bytes;
-// LB: LDS bytes per stored element (both planes for F32X3); RP: row blocks per item.
+// KA: A elements per lane-half per step; ESZ: global element bytes; LB: LDS bytes
+// per stored element (both planes for F32X3); RP: row blocks per wave item;
+// PF: k-steps per prefetch chunk; FS: packed bytes per lane per fragment.
 template <> struct CT<K_F32> {
-  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, LB = 4, RP = 1, PF = 4;
+  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, LB = 4, RP = 1, PF = 4, FS = 16;
   using Acc = f32x16;
 };
 template <> struct CT<K_BF16> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 2, LB = 2, RP = 2, PF = 4;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 2, LB = 2, RP = 2, PF = 4, FS = 16;
   using Acc = f32x16;
 };
 template <> struct CT<K_I8> {
-  static constexpr int KG = 16, KC = 32, KA = 16, ESZ = 1, LB = 1, RP = 2, PF = 4;
+  static constexpr int KG = 16, KC = 32, KA = 16, ESZ = 1, LB = 1, RP = 2, PF = 4, FS = 16;
   using Acc = i32x16;
 };
 template <> struct CT<K_F32X3> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, LB = 4, RP = 1, PF = 2;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, LB = 4, RP = 1, PF = 2, FS = 32;
   using Acc = f32x16;
 };
 
 template <int TN>
 __device__ __forceinline__ int cslot(int g, int n) { return g * TN + (n ^ (g & 7)); }
 
-// One lane's A fragment for one k-step: 16 bytes (32 for F32X3, as fp32).
-template <int KIND> struct AFrag { uint4 v[KIND == K_F32X3 ? 2 : 1]; };
-
-// One lane's A fragment for k-step `ks`: row `rowp` (a clamped, always valid row
-// pointer) at element k = ks*KC + h*KA. The fused path requires K*ESZ % 16 == 0 and
-// 16-byte aligned A (the planner sends other shapes to the unfused path), so a
-// fragment lies entirely inside or entirely past K. Out-of-range lanes (row >= M,
-// k >= K) read a clamped in-bounds address and are zeroed with a select: no
-// divergent branches in the k loop.
+// --------------------------------------------------------- packed weights --
+// Packed layout of A (M x K) for kind KIND: fragments f = (rg * kspad + ks) * RP + i
+// for row group rg (RP * 32 rows), k-step ks < kspad and row block i; fragment f is
+// 64 lanes x FS bytes at f * 64 * FS, lane l = h*32 + r holding row
+// (rg*RP + i)*32 + r, elements k = ks*KC + h*KA .. + KA-1 (zero outside A). For
+// F32X3 the fragment is the 1 KB hi half then the 1 KB lo half.
 template <int KIND>
-__device__ __forceinline__ AFrag<KIND> load_afrag(const char *rowp, long K, long k, bool row_ok) {
+__host__ __device__ inline long kspad_of(long K) {
   using T = CT<KIND>;
-  constexpr int NV = KIND == K_F32X3 ? 2 : 1;
-  const bool ok = row_ok && (k < K);
-  const long kc = ok ? k : 0;
-  const char *p = rowp + kc * T::ESZ;
-  AFrag<KIND> f;
-#pragma unroll
-  for (int q = 0; q < NV; ++q) {
-    uint4 v = *(const uint4 *)(p + 16 * q);
-    if (KIND == K_I8) {  // u8 -> s8 bias: x ^ 0x80 = x - 128 as signed
-      v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
-    }
-    f.v[q].x = ok ? v.x : 0u;
-    f.v[q].y = ok ? v.y : 0u;
-    f.v[q].z = ok ? v.z : 0u;
-    f.v[q].w = ok ? v.w : 0u;
-  }
-  return f;
+  const long ks = (K + T::KC - 1) / T::KC;
+  return (ks + T::PF - 1) / T::PF * T::PF;
 }
 
-__device__ __forceinline__ uint32_t pk(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+template <int KIND>
+__host__ __device__ inline long packed_frags(long M, long K) {
+  using T = CT<KIND>;
+  const long groups = (M + 32 * T::RP - 1) / (32 * T::RP);
+  return groups * kspad_of<KIND>(K) * T::RP;
+}
+
+// One lane of packed fragment f of A.
+template <int KIND>
+__device__ __forceinline__ void pack_frag(const char *__restrict__ A, long M, long K, char *__restrict__ out, long f,
+                                          int lane) {
+  using T = CT<KIND>;
+  const int h = lane >> 5, r = lane & 31;
+  const long kspad = kspad_of<KIND>(K);
+  const long i = f % T::RP, ks = (f / T::RP) % kspad, rg = f / (T::RP * kspad);
+  const long row = (rg * T::RP + i) * 32 + r;
+  const long k0 = ks * T::KC + h * T::KA;
+  uint32_t w[T::KA * T::ESZ / 4];
+#pragma unroll
+  for (int d = 0; d < T::KA * T::ESZ / 4; ++d) w[d] = 0u;
+  if (row < M) {
+    const char *p = A + ((size_t)row * K + k0) * T::ESZ;
+#pragma unroll
+    for (int e = 0; e < T::KA; ++e)
+      if (k0 + e < K) {
+        uint32_t x;
+        if (T::ESZ == 4) x = *(const uint32_t *)(p + 4 * e);
+        else if (T::ESZ == 2) x = *(const uint16_t *)(p + 2 * e);
+        else x = (uint32_t)(*(const uint8_t *)(p + e) ^ 0x80u);  // u8 -> s8 bias
+        w[(e * T::ESZ) >> 2] |= x << (((e * T::ESZ) & 3) * 8);
+      }
+  }
+  char *dst = out + f * 64 * T::FS + lane * 16;
+  if (KIND == K_F32X3) {
+    uint32_t hi[4], lo[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      split_pk(__builtin_bit_cast(float, w[2 * e]), __builtin_bit_cast(float, w[2 * e + 1]), hi[e], lo[e]);
+    *(uint4 *)dst = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    *(uint4 *)(dst + 1024) = make_uint4(lo[0], lo[1], lo[2], lo[3]);
+  } else {
+    *(uint4 *)dst = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+// Up to PACK_MAX matrices packed by one launch: fragment f belongs to matrix m when
+// fstart[m] <= f < fstart[m+1]; four fragments (one per wave) per workgroup.
+template <int KIND>
+__global__ __launch_bounds__(256) void pack_a_kernel(PackJob J) {
+  const long f = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (f >= J.fstart[J.n]) return;
+  int m = 0;
+  while (m + 1 < J.n && f >= J.fstart[m + 1]) ++m;
+  pack_frag<KIND>((const char *)J.A[m], J.M[m], J.K[m], (char *)J.out[m], f - J.fstart[m], threadIdx.x & 63);
+}
+
+// One lane's packed A fragment: 16 bytes, or hi and lo halves for F32X3.
+template <int KIND> struct AFrag { uint4 v[KIND == K_F32X3 ? 2 : 1]; };
 
 // One lane's B fragments for one k-step: TNB column blocks (hi and lo planes for
 // F32X3), read from the arena one step ahead of their MFMAs.
@@ -103,51 +149,30 @@ template <int KIND, int TNB>
 __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KIND> (&af)[CT<KIND>::RP],
                                       typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
   constexpr int RP = CT<KIND>::RP;
-  if (KIND == K_F32X3) {
-    uint4 ah[RP], al[RP];
-#pragma unroll
-    for (int i = 0; i < RP; ++i) {
-      const uint32_t w[8] = {af[i].v[0].x, af[i].v[0].y, af[i].v[0].z, af[i].v[0].w,
-                             af[i].v[1].x, af[i].v[1].y, af[i].v[1].z, af[i].v[1].w};
-      uint32_t h[4], l[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e)
-        split_pk(__builtin_bit_cast(float, w[2 * e]), __builtin_bit_cast(float, w[2 * e + 1]), h[e], l[e]);
-      ah[i] = make_uint4(h[0], h[1], h[2], h[3]);
-      al[i] = make_uint4(l[0], l[1], l[2], l[3]);
-    }
-#pragma unroll
-    for (int j = 0; j < TNB; ++j) {
-      const uint4 bh = bf.b[j], bl = bf.lo[j];
-#pragma unroll
-      for (int i = 0; i < RP; ++i) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al[i]),
-                                                            __builtin_bit_cast(bf16x8, bh), acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[i]),
-                                                            __builtin_bit_cast(bf16x8, bl), acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah[i]),
-                                                            __builtin_bit_cast(bf16x8, bh), acc[i][j], 0, 0, 0);
-      }
-    }
-    return;
-  }
 #pragma unroll
   for (int j = 0; j < TNB; ++j) {
-    const uint4 b = bf.b[j];
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
-      if (KIND == K_F32) {
-        f32x4 av = __builtin_bit_cast(f32x4, af[i].v[0]);
-        f32x4 bv = __builtin_bit_cast(f32x4, b);
+      if (KIND == K_F32X3) {
+        // small terms first, then the dominant hi.hi product
+        const bf16x8 ah = __builtin_bit_cast(bf16x8, af[i].v[0]), al = __builtin_bit_cast(bf16x8, af[i].v[1]);
+        const bf16x8 bh = __builtin_bit_cast(bf16x8, bf.b[j]), bl = __builtin_bit_cast(bf16x8, bf.lo[j]);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[i][j], 0, 0, 0);
+      } else if (KIND == K_F32) {
+        // lane half h holds k = kc + 4h + s for s = 0..3 in both operands
+        const f32x4 av = __builtin_bit_cast(f32x4, af[i].v[0]);
+        const f32x4 bv = __builtin_bit_cast(f32x4, bf.b[j]);
 #pragma unroll
         for (int s = 0; s < 4; ++s)
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s], bv[s], acc[i][j], 0, 0, 0);
       } else if (KIND == K_BF16) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, af[i].v[0]),
-                                                            __builtin_bit_cast(bf16x8, b), acc[i][j], 0, 0, 0);
+                                                            __builtin_bit_cast(bf16x8, bf.b[j]), acc[i][j], 0, 0, 0);
       } else {
         acc[i][j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(__builtin_bit_cast(i32x4, af[i].v[0]),
-                                                          __builtin_bit_cast(i32x4, b), acc[i][j], 0, 0, 0);
+                                                          __builtin_bit_cast(i32x4, bf.b[j]), acc[i][j], 0, 0, 0);
       }
     }
   }
@@ -237,7 +262,8 @@ __device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int *colsum)
   }
 }
 
-// One workgroup runs the whole chain on columns [n0, n0 + TN) of B.
+// One workgroup runs the whole chain on columns [n0, n0 + TN) of B. P.A[i] are
+// packed weights (pack_a_kernel layout for KIND).
 template <int KIND, int TNB>
 __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
@@ -245,6 +271,7 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   constexpr int TN = TNB * 32;
   constexpr int RP = T::RP;
   constexpr int PF = T::PF;
+  constexpr int FRAG = 64 * T::FS;  // bytes per packed fragment
   const int lo_off = (KIND == K_F32X3) ? (P.S / 8) * TN : 0;  // lo plane (F32X3)
   int *colsum = (int *)(arena + (size_t)(P.S / T::KG) * TN * (KIND == K_F32X3 ? 2 : 1));
 
@@ -261,16 +288,15 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
     const long Mp = (M + 31) / 32 * 32;
     const int xg0 = P.xbase[gi] / T::KG;
     const bool last = gi == P.num_gemm - 1;
-    const char *A = (const char *)P.A[gi];
     if (KIND == K_I8) {
       colsum_lds<TN>(arena, xg0, (int)((K + 31) / 32 * 32 / T::KG), colsum);
       __syncthreads();
     }
-    // k-steps: `full` ones have every lane's fragment inside K; they run in chunks of
-    // PF with no bounds logic. The remaining steps (K tail) use checked loads.
-    const int ksteps = (int)((K + T::KC - 1) / T::KC);
-    const int kfull = (int)(K / T::KC);
-    const int nfc = kfull / PF;  // full chunks
+    // k-steps padded to whole chunks (the packed A is zero there); B reads past the
+    // X region clamp to its last group, which then meets zero A.
+    const int kspad = (int)kspad_of<KIND>(K);
+    const int nchunks = kspad / PF;
+    const int glast = xg0 + (int)((K + 31) / 32 * 32 / T::KG) - 1;
     // INT8: the accumulators start at the bias 128 * sum_k X[k][n] of their column,
     // so the MFMAs accumulate u8(a).b directly.
     int bias0[TNB];
@@ -285,41 +311,23 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
 #pragma unroll
           for (int r = 0; r < 16; ++r) acc[i][j][r] = bias0[j];
 
-      // Per-lane row pointers clamped into the matrix. Rows >= M compute garbage
-      // that is never used: the last GEMM does not store them, and an intermediate's
-      // padded rows meet zero A columns in the next GEMM (INT8 zeroes them anyway,
-      // because they enter its column sums).
-      const char *rowp[RP];
-      bool row_ok[RP];
-#pragma unroll
-      for (int i = 0; i < RP; ++i) {
-        const long row = rb0 + i * 32 + l32;
-        row_ok[i] = row < M;
-        rowp[i] = A + ((size_t)(row_ok[i] ? row : M - 1) * K + h * T::KA) * T::ESZ;
-      }
+      // this wave's row group in the packed weights; fragment (ks, i) at
+      // apk + (ks * RP + i) * FRAG, this lane's 16 bytes at + lane * 16
+      const char *apk = (const char *)P.A[gi] + ((size_t)(rb0 / (32 * RP)) * kspad * RP) * FRAG + lane * 16;
       AFrag<KIND> fa[2][PF][RP];
       auto load_chunk = [&](int c, AFrag<KIND>(&dst)[PF][RP]) {
 #pragma unroll
-        for (int s = 0; s < PF; ++s) {
-          const size_t off = (size_t)(c * PF + s) * T::KC * T::ESZ;
+        for (int s = 0; s < PF; ++s)
 #pragma unroll
           for (int i = 0; i < RP; ++i) {
-#pragma unroll
-            for (int q = 0; q < (KIND == K_F32X3 ? 2 : 1); ++q) {
-              uint4 v = *(const uint4 *)(rowp[i] + off + 16 * q);
-              if (KIND == K_I8) {  // u8 -> s8 bias
-                v.x ^= 0x80808080u; v.y ^= 0x80808080u; v.z ^= 0x80808080u; v.w ^= 0x80808080u;
-              }
-              dst[s][i].v[q] = v;
-            }
+            const char *p = apk + ((size_t)(c * PF + s) * RP + i) * FRAG;
+            dst[s][i].v[0] = *(const uint4 *)p;
+            if (KIND == K_F32X3) dst[s][i].v[1] = *(const uint4 *)(p + 1024);
           }
-        }
       };
       // B fragments one step ahead, double buffered (PF is even, so the buffer of
-      // step ks is ks & 1 across chunks). The read for the step after the last
-      // clamps to the last step: harmless and branch free.
+      // step ks is ks & 1 across chunks).
       BFrag<KIND, TNB> bq[2];
-      const int glast = xg0 + 2 * (ksteps - 1) + h;
       auto run_chunk = [&](int c, const AFrag<KIND>(&src)[PF][RP]) {
 #pragma unroll
         for (int s = 0; s < PF; ++s) {
@@ -329,25 +337,15 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
           mma_k<KIND, TNB>(bq[s & 1], src[s], acc);
         }
       };
-      read_bfrag<KIND, TNB>(arena, lo_off, xg0 + h, l32, bq[0]);
-      if (nfc > 0) load_chunk(0, fa[0]);
-      for (int c = 0; c < nfc; c += 2) {
-        if (c + 1 < nfc) load_chunk(c + 1, fa[1]);
+      read_bfrag<KIND, TNB>(arena, lo_off, (xg0 + h) < glast ? xg0 + h : glast, l32, bq[0]);
+      load_chunk(0, fa[0]);
+      for (int c = 0; c < nchunks; c += 2) {
+        if (c + 1 < nchunks) load_chunk(c + 1, fa[1]);
         run_chunk(c, fa[0]);
-        if (c + 1 < nfc) {
-          if (c + 2 < nfc) load_chunk(c + 2, fa[0]);
+        if (c + 1 < nchunks) {
+          if (c + 2 < nchunks) load_chunk(c + 2, fa[0]);
           run_chunk(c + 1, fa[1]);
         }
-      }
-      // K tail: checked loads, one step at a time
-      for (int ks = nfc * PF; ks < ksteps; ++ks) {
-        AFrag<KIND> af[RP];
-        const long k = (long)ks * T::KC + h * T::KA;
-#pragma unroll
-        for (int i = 0; i < RP; ++i) af[i] = load_afrag<KIND>(rowp[i] - h * T::KA * T::ESZ, K, k, row_ok[i]);
-        BFrag<KIND, TNB> bt;
-        read_bfrag<KIND, TNB>(arena, lo_off, xg0 + 2 * ks + h, l32, bt);
-        mma_k<KIND, TNB>(bt, af, acc);
       }
 
       // epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32
@@ -413,7 +411,6 @@ __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   chain_tile<KIND, TNB>(P, (long)blockIdx.x * (TNB * 32), arena);
 }
 
-constexpr int KG_of(int kind) { return kind == K_F32 ? 4 : kind == K_BF16 ? 8 : kind == K_I8 ? 16 : 8; }
 constexpr int LB_of(int kind) { return kind == K_F32 ? 4 : kind == K_BF16 ? 2 : kind == K_I8 ? 1 : 4; }
 
 // Arena rows and region placement. X_0 (the B tile) sits at row 0; GEMM i writes
@@ -498,6 +495,16 @@ hipError_t launch_chain_k(const ChainArgs &c, hipStream_t s) {
   return hipErrorInvalidValue;
 }
 
+template <int KIND>
+hipError_t launch_pack_t(PackJob J, hipStream_t s) {
+  J.fstart[0] = 0;
+  for (int m = 0; m < J.n; ++m) J.fstart[m + 1] = J.fstart[m] + packed_frags<KIND>(J.M[m], J.K[m]);
+  const long nf = J.fstart[J.n];
+  if (nf == 0) return hipSuccess;
+  hipLaunchKernelGGL((pack_a_kernel<KIND>), dim3((unsigned)((nf + 3) / 4)), dim3(256), 0, s, J);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n) {
@@ -509,6 +516,41 @@ size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gem
 
 int chain_fused_max_gemms() { return MAXG; }
 
+size_t packed_a_bytes(int kind, long M, long K) {
+  switch (kind) {
+    case K_F32: return (size_t)packed_frags<K_F32>(M, K) * 64 * CT<K_F32>::FS;
+    case K_BF16: return (size_t)packed_frags<K_BF16>(M, K) * 64 * CT<K_BF16>::FS;
+    case K_I8: return (size_t)packed_frags<K_I8>(M, K) * 64 * CT<K_I8>::FS;
+    case K_F32X3: return (size_t)packed_frags<K_F32X3>(M, K) * 64 * CT<K_F32X3>::FS;
+  }
+  return 0;
+}
+
+hipError_t launch_pack_a(int kind, const PackJob &J, hipStream_t s) {
+  if (J.n < 1 || J.n > PACK_MAX) return hipErrorInvalidValue;
+  for (int m = 0; m < J.n; ++m)
+    if (J.M[m] <= 0 || J.K[m] <= 0) return hipErrorInvalidValue;
+  switch (kind) {
+    case K_F32: return launch_pack_t<K_F32>(J, s);
+    case K_BF16: return launch_pack_t<K_BF16>(J, s);
+    case K_I8: return launch_pack_t<K_I8>(J, s);
+    case K_F32X3: return launch_pack_t<K_F32X3>(J, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s) {
+  if (c.num_gemm < 1 || c.num_gemm > MAXG || c.N <= 0) return hipErrorInvalidValue;
+  if (chain_fused_lds_bytes(c.kind, c.M, c.K, c.num_gemm, c.tile_n) > LDS_LIMIT) return hipErrorInvalidValue;
+  switch (c.kind) {
+    case K_F32: return launch_chain_k<K_F32>(c, s);
+    case K_BF16: return launch_chain_k<K_BF16>(c, s);
+    case K_I8: return launch_chain_k<K_I8>(c, s);
+    case K_F32X3: return launch_chain_k<K_F32X3>(c, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 void make_chain_params(const ChainArgs &c, ChainParams *P) {
   static const int esz[4] = {4, 2, 1, 4};
   const int e = esz[c.kind];
@@ -519,7 +561,6 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
     P->A[i] = c.A[i];
     P->M[i] = c.M[i];
     P->K[i] = c.K[i];
-    P->vecA[i] = ((c.K[i] * e) % 16 == 0) && (((uintptr_t)c.A[i] & 15) == 0);
   }
   P->S = plan_arena(c.M, c.K, c.num_gemm, P->xbase, P->ybase);
   P->B = c.B;
@@ -529,19 +570,12 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
 }
 
 bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K) {
-  static const int esz[4] = {4, 2, 1, 4};
-  if (num_gemm < 1 || num_gemm > MAXG || N % 4 != 0) return false;
-  for (int i = 0; i < num_gemm; ++i)
-    if ((K[i] * esz[kind]) % 16 != 0) return false;
-  return true;
+  (void)kind;
+  (void)K;
+  return num_gemm >= 1 && num_gemm <= MAXG && N % 4 == 0;
 }
 
-bool chain_fused_aligned(const ChainParams &P) {
-  if (!P.vecB) return false;
-  for (int i = 0; i < P.num_gemm; ++i)
-    if (!P.vecA[i]) return false;
-  return true;
-}
+bool chain_fused_aligned(const ChainParams &P) { return P.vecB != 0; }
 
 int grouped_tile_n(int kind) {
   switch (kind) {
@@ -560,18 +594,6 @@ hipError_t launch_grouped(const void *table, int grid, size_t lds, hipStream_t s
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(grouped_chain_kernel, dim3(grid), dim3(NTH), lds, s, (const char *)table);
   return hipGetLastError();
-}
-
-hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s) {
-  if (c.num_gemm < 1 || c.num_gemm > MAXG || c.N <= 0) return hipErrorInvalidValue;
-  if (chain_fused_lds_bytes(c.kind, c.M, c.K, c.num_gemm, c.tile_n) > LDS_LIMIT) return hipErrorInvalidValue;
-  switch (c.kind) {
-    case K_F32: return launch_chain_k<K_F32>(c, s);
-    case K_BF16: return launch_chain_k<K_BF16>(c, s);
-    case K_I8: return launch_chain_k<K_I8>(c, s);
-    case K_F32X3: return launch_chain_k<K_F32X3>(c, s);
-  }
-  return hipErrorInvalidValue;
 }
 
 }  // namespace fg

@@ -155,7 +155,7 @@ constexpr int CHAIN_MAXG = 8;
 struct ChainArgs {
   int kind;                 // Kind
   int num_gemm;
-  const void *A[CHAIN_MAXG];
+  const void *A[CHAIN_MAXG];  // packed weights for the fused kernels
   long M[CHAIN_MAXG], K[CHAIN_MAXG];
   const void *B;            // K1 x N
   void *C;                  // M_n x N (fp32 or int32)
@@ -165,6 +165,7 @@ struct ChainArgs {
 
 // Device-side description of one chain for the fused kernels (filled on the host
 // by make_chain_params; copied by value or read from a table in device memory).
+// A[i] are packed weights (launch_pack_a layout for `kind`).
 struct ChainParams {
   int kind;                 // Kind
   const void *A[CHAIN_MAXG];
@@ -176,15 +177,27 @@ struct ChainParams {
   const void *B;
   void *C;
   long N;
-  int vecA[CHAIN_MAXG];
   int vecB;
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);
-// The fused kernels need every A row and the B tile in whole 16-byte / 4-element
-// groups: K_i * elem % 16 == 0 and N % 4 == 0 (shape), and aligned base pointers.
+// The fused kernels stage B in groups of 4 columns: N % 4 == 0 (shape) and a B
+// base aligned to 4 elements.
 bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K);
 bool chain_fused_aligned(const ChainParams &P);
+
+// Weights in MFMA fragment order for the fused kernels (zero padded; INT8 biased
+// to s8; F32X3 split into bf16 hi/lo). packed_a_bytes is the buffer size.
+size_t packed_a_bytes(int kind, long M, long K);
+constexpr int PACK_MAX = 32;
+struct PackJob {  // up to PACK_MAX matrices of one kind in one launch
+  const void *A[PACK_MAX];
+  void *out[PACK_MAX];
+  long M[PACK_MAX], K[PACK_MAX];
+  long fstart[PACK_MAX + 1];  // filled by the launcher
+  int n;
+};
+hipError_t launch_pack_a(int kind, const PackJob &J, hipStream_t s);
 
 // Fused chain: every intermediate stays in LDS. Returns hipErrorInvalidValue when the
 // largest (K_i + M_i) x tile_n panel does not fit the LDS budget.

@@ -60,9 +60,65 @@ int describe(const fg_chain_req &r, ChainParams *P, int *tile_n, size_t *lds, bo
   return FG_OK;
 }
 
-// Pack the requests that fit into a table; `rest` receives the indices of the others.
-int pack(int count, const fg_chain_req *reqs, Packed *out, std::vector<int> *rest) {
+// Weights of requests that are not pre-packed are packed on stream s into its
+// "gpack" scratch before the grouped kernel runs: each distinct (kind, A, M, K) once,
+// one launch per kind and PACK_MAX matrices. P[j].A then point at the packed copies.
+int pack_weights(const fg_chain_req *reqs, const std::vector<int> &src, std::vector<ChainParams> &P,
+                 hipStream_t s) {
+  struct W {
+    int kind;
+    const void *A;
+    long M, K;
+    size_t off;
+  };
+  std::vector<W> ws;
+  std::vector<std::vector<int>> slot(P.size());  // P[j].A[i] -> ws index
+  size_t total = 0;
+  for (size_t j = 0; j < P.size(); ++j) {
+    if (reqs[src[j]].a_packed) continue;
+    slot[j].assign(P[j].num_gemm, -1);
+    for (int i = 0; i < P[j].num_gemm; ++i) {
+      int found = -1;
+      for (size_t w = 0; w < ws.size() && found < 0; ++w)
+        if (ws[w].kind == P[j].kind && ws[w].A == P[j].A[i] && ws[w].M == P[j].M[i] && ws[w].K == P[j].K[i])
+          found = (int)w;
+      if (found < 0) {
+        found = (int)ws.size();
+        ws.push_back({P[j].kind, P[j].A[i], P[j].M[i], P[j].K[i], total});
+        total += (packed_a_bytes(P[j].kind, P[j].M[i], P[j].K[i]) + 255) & ~(size_t)255;
+      }
+      slot[j][i] = found;
+    }
+  }
+  if (ws.empty()) return FG_OK;
+  char *buf = nullptr;
+  int rc = scratch(s, "gpack", total, (void **)&buf);
+  if (rc) return rc;
+  for (int kind = 0; kind < 4; ++kind) {
+    PackJob J{};
+    for (size_t w = 0; w <= ws.size(); ++w) {
+      if (J.n == PACK_MAX || (w == ws.size() && J.n > 0)) {
+        if ((rc = hip_status(launch_pack_a(kind, J, s), "pack weights"))) return rc;
+        J.n = 0;
+      }
+      if (w == ws.size() || ws[w].kind != kind) continue;
+      J.A[J.n] = ws[w].A;
+      J.M[J.n] = ws[w].M;
+      J.K[J.n] = ws[w].K;
+      J.out[J.n] = buf + ws[w].off;
+      ++J.n;
+    }
+  }
+  for (size_t j = 0; j < P.size(); ++j)
+    for (size_t i = 0; i < slot[j].size(); ++i) P[j].A[i] = buf + ws[slot[j][i]].off;
+  return FG_OK;
+}
+
+// Pack the requests that fit into a table (packing their weights on stream s when
+// needed); `rest` receives the indices of the others.
+int pack(int count, const fg_chain_req *reqs, hipStream_t s, Packed *out, std::vector<int> *rest) {
   std::vector<ChainParams> P;
+  std::vector<int> src;  // request index of P[j]
   struct Item {
     int req, tile;
     double cost;
@@ -85,6 +141,7 @@ int pack(int count, const fg_chain_req *reqs, Packed *out, std::vector<int> *res
     for (int i = 0; i < p.num_gemm; ++i) per_col += (double)p.M[i] * p.K[i];
     int idx = (int)P.size();
     P.push_back(p);
+    src.push_back(r);
     long tiles = (reqs[r].N + tn - 1) / tn;
     for (long t = 0; t < tiles; ++t) {
       long w = std::min<long>(tn, reqs[r].N - t * tn);
@@ -92,6 +149,7 @@ int pack(int count, const fg_chain_req *reqs, Packed *out, std::vector<int> *res
     }
   }
   std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+  if (int rc = pack_weights(reqs, src, P, s)) return rc;
   GroupTable h{};
   h.num_reqs = (int)P.size();
   h.num_items = (int)items.size();
@@ -176,7 +234,7 @@ int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   Packed pk;
   std::vector<int> rest;
-  if ((rc = pack(count, reqs, &pk, &rest))) return rc;
+  if ((rc = pack(count, reqs, s, &pk, &rest))) return rc;
   if (pk.items > 0) {
     StreamRing *R = ring_for(s);
     Slot &sl = R->slot[R->next];
@@ -193,6 +251,7 @@ int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream) {
     fg_chain_opts o;
     fg_chain_opts_default(&o);
     o.f32_mode = q.f32_mode;
+    o.a_packed = q.a_packed;
     if ((rc = fg_chain(q.dtype, q.C, q.B, q.N, q.num_gemm, q.A, q.M, q.K, &o, stream))) return rc;
   }
   return FG_OK;
@@ -258,7 +317,7 @@ int fg_grouped_graph_launch(fg_grouped_graph *G, int count, const fg_chain_req *
   if (!G || count < 0 || (count > 0 && !reqs)) return set_error(FG_ERR_INVALID, "graph launch");
   Packed pk;
   std::vector<int> rest;
-  if ((rc = pack(count, reqs, &pk, &rest))) return rc;
+  if ((rc = pack(count, reqs, (hipStream_t)stream, &pk, &rest))) return rc;
   if (!rest.empty())
     return set_error(FG_ERR_UNSUPPORTED, "request %d does not fit on chip at the grouped tile width", rest[0]);
   if (pk.bytes.size() > G->table_cap) return set_error(FG_ERR_INVALID, "batch exceeds the graph's table");

@@ -36,9 +36,10 @@ namespace {
 
 struct WeightSet {
   int dtype, f32_mode, num_gemm;
-  void *A[CHAIN_MAXG];
+  void *A[CHAIN_MAXG];       // the caller's weights (unfused and per-GEMM runs)
+  void *packed[CHAIN_MAXG];  // packed once at registration (grouped fused runs)
   long M[CHAIN_MAXG], K[CHAIN_MAXG];
-  size_t bytes;
+  size_t bytes;              // raw + packed bytes, charged to the budget
 };
 
 struct Request {
@@ -181,7 +182,13 @@ bool step(fg_server *S) {
         x.M[g] = w.M[g];
         x.K[g] = w.K[g];
       }
-      (fg_grouped_fits(&x) == 1 ? fit : rest).push_back(x);
+      if (fg_grouped_fits(&x) == 1) {
+        for (int g = 0; g < w.num_gemm; ++g) x.A[g] = w.packed[g];
+        x.a_packed = 1;
+        fit.push_back(x);
+      } else {
+        rest.push_back(x);
+      }
     }
     if (!fit.empty()) {
       if (S->cfg.use_graph)
@@ -262,17 +269,40 @@ int fg_server_add_weights(fg_server *S, int dtype, int f32_mode, int num_gemm, c
   w.f32_mode = f32_mode;
   w.num_gemm = num_gemm;
   size_t e = dtype == FG_DTYPE_F32 ? 4 : dtype == FG_DTYPE_BF16 ? 2 : 1;
+  const bool pack = !S->cfg.per_gemm_baseline;
   for (int i = 0; i < num_gemm; ++i) {
-    if (M[i] <= 0 || K[i] <= 0 || (i > 0 && K[i] != M[i - 1])) return set_error(FG_ERR_INVALID, "weight shapes");
+    if (!A[i] || M[i] <= 0 || K[i] <= 0 || (i > 0 && K[i] != M[i - 1]))
+      return set_error(FG_ERR_INVALID, "weight shapes");
     w.A[i] = (void *)A[i];
     w.M[i] = M[i];
     w.K[i] = K[i];
     w.bytes += (size_t)M[i] * K[i] * e;
+    if (pack) w.bytes += fg_packed_weight_bytes(dtype, f32_mode, M[i], K[i]);
+  }
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->weight_bytes + w.bytes > S->cfg.hbm_budget_bytes)
+      return set_error(FG_ERR_BUDGET, "weights exceed the HBM budget");
+    S->weight_bytes += w.bytes;  // reserved while packing
+  }
+  int rc = FG_OK;
+  if (pack) {
+    // pack once on the first server stream; registration waits for it
+    hipStream_t st = S->streams[0];
+    for (int i = 0; i < num_gemm && !rc; ++i) {
+      rc = hip_status(hipMalloc(&w.packed[i], fg_packed_weight_bytes(dtype, f32_mode, M[i], K[i])),
+                      "hipMalloc(packed weights)");
+      if (!rc) rc = fg_pack_weight(dtype, f32_mode, A[i], M[i], K[i], w.packed[i], st);
+    }
+    if (!rc) rc = hip_status(hipStreamSynchronize(st), "hipStreamSynchronize");
   }
   std::lock_guard<std::mutex> lk(S->mu);
-  if (S->weight_bytes + w.bytes > S->cfg.hbm_budget_bytes)
-    return set_error(FG_ERR_BUDGET, "weights exceed the HBM budget");
-  S->weight_bytes += w.bytes;
+  if (rc) {
+    for (int i = 0; i < num_gemm; ++i)
+      if (w.packed[i]) (void)hipFree(w.packed[i]);
+    S->weight_bytes -= w.bytes;
+    return rc;
+  }
   *set_id = (int)S->sets.size();
   S->sets.push_back(w);
   return FG_OK;
@@ -354,6 +384,9 @@ void fg_server_destroy(fg_server *S) {
   for (auto e : S->free_events) (void)hipEventDestroy(e);
   for (auto g : S->graphs) fg_grouped_graph_destroy(g);
   for (auto s : S->streams) (void)hipStreamDestroy(s);
+  for (auto &w : S->sets)
+    for (int i = 0; i < w.num_gemm; ++i)
+      if (w.packed[i]) (void)hipFree(w.packed[i]);
   delete S;
 }
 

@@ -84,6 +84,7 @@ typedef struct fg_chain_opts {
   int fuse;        /* FG_FUSE_AUTO (default), FG_FUSE_OFF, FG_FUSE_ON    */
   int tile_n;      /* fused tile width: 0 = planner, or 32, 64, 96, 128  */
   int splits;      /* split-K of unfused GEMMs: 0 = planner              */
+  int a_packed;    /* 1: A[i] come from fg_pack_weight (fused path only) */
 } fg_chain_opts;
 
 void fg_chain_opts_default(fg_chain_opts *opts);
@@ -112,6 +113,15 @@ int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, i
              void *stream);
 int fg_sgemm_host(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode);
 
+/* Weights in the fused kernels' packed layout: MFMA fragment order, zero padded,
+ * INT8 bytes pre-biased to signed, FP32 for FG_F32_BF16X3 pre-split into bf16 hi/lo.
+ * The reference packs A inside every call (multigemm/src/PACK_x86.h:616-647,
+ * flashgemm.c:85-105); fg_chain does the same unless opts->a_packed says A[i] are
+ * already packed (for weights that stay resident). packed must hold
+ * fg_packed_weight_bytes bytes; the layout depends on dtype and f32_mode. */
+size_t fg_packed_weight_bytes(int dtype, int f32_mode, long M, long K);
+int fg_pack_weight(int dtype, int f32_mode, const void *A, long M, long K, void *packed, void *stream);
+
 /* Plan inspection: writes the planner's decision for a chain without running it.
  * fused = 1 if the whole chain runs in one fused launch, tile_n its tile width. */
 int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
@@ -129,6 +139,7 @@ typedef struct fg_chain_req {
   int num_gemm;          /* 1..8                              */
   const void *A[8];      /* A_i, M_i x K_i                    */
   long M[8], K[8];
+  int a_packed;          /* 1: A[i] come from fg_pack_weight  */
 } fg_chain_req;
 
 /* All requests in one persistent launch: each request's columns are cut into tiles

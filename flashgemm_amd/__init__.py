@@ -27,6 +27,8 @@ DTYPE_F32, DTYPE_BF16, DTYPE_INT8 = 0, 1, 2
 F32_MFMA, F32_BF16X3 = 0, 1
 LAYOUT_MATH, LAYOUT_REFERENCE = 0, 1
 FUSE_AUTO, FUSE_OFF, FUSE_ON = -1, 0, 1
+(OK, ERR_INVALID, ERR_DOMAIN, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_NODEVICE,
+ ERR_UNSUPPORTED) = 0, -1, -2, -3, -4, -5, -6, -7
 
 STATUS = {0: "FG_OK", -1: "FG_ERR_INVALID", -2: "FG_ERR_DOMAIN", -3: "FG_ERR_NOMEM", -4: "FG_ERR_HIP",
           -5: "FG_ERR_BUDGET", -6: "FG_ERR_NODEVICE", -7: "FG_ERR_UNSUPPORTED"}

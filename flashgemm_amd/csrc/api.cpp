@@ -110,9 +110,10 @@ int plan_splits(long M, long N, long K, int kind) {
 int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K) {
   if (num_gemm > chain_fused_max_gemms() || !chain_fused_shape_ok(kind, N, num_gemm, K)) return 0;
   const int cand[4] = {128, 96, 64, 32};
+  const int widest = chain_fused_max_tile(kind);
   int best = 0;
   for (int c : cand) {
-    if (chain_fused_lds_bytes(kind, M, K, num_gemm, c) > 160 * 1024) continue;
+    if (c > widest || chain_fused_lds_bytes(kind, M, K, num_gemm, c) > 160 * 1024) continue;
     long tiles = (N + c - 1) / c;
     if (tiles >= num_cus()) return c;
     best = c;  // keep shrinking for more workgroups
@@ -250,7 +251,7 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   int tn = 0;
   if (!ref && o.fuse != FG_FUSE_OFF) {
     if (o.tile_n > 0) {
-      if (chain_fused_shape_ok(kind, N, num_gemm, K) &&
+      if (o.tile_n <= chain_fused_max_tile(kind) && chain_fused_shape_ok(kind, N, num_gemm, K) &&
           chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n) <= LDS_BYTES_MAX)
         tn = o.tile_n;
     } else {

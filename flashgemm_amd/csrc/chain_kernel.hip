@@ -12,12 +12,15 @@
 //     bounds logic), INT8 bytes carry the u8 -> s8 bias (x ^ 0x80) and FP32 for the
 //     split-bf16 form is stored as its bf16 hi/lo halves;
 //   * the B tile (K1 x TN) is staged once into an LDS arena in the MFMA B-operand
-//     layout (16-byte k-groups per column, slot g*TN + (n ^ (g & 7)));
-//   * GEMM i reads X_i from one end of the arena and writes its requantized output
-//     Y_i (fp32 / bf16 RNE / int8 saturate) at the other end, so X_{i+1} = Y_i never
-//     leaves the CU; the arena holds max_i (K_i + M_i) rows;
-//   * each wave owns row blocks of the weights, streams their packed fragments PF
-//     k-steps ahead (double buffered) and reads B fragments one step ahead;
+//     layout (16-byte k-groups per column, slot g*TN + n);
+//   * GEMM i reads X_i and writes its requantized output Y_i (fp32 / bf16 RNE /
+//     int8 saturate) right after it in a circular arena, so X_{i+1} = Y_i never
+//     leaves the CU; each wave holds its final row pass in registers until every
+//     wave is done with X_i, which lets Y_i reuse X_i's rows: the arena needs
+//     max_i (K_i + M_i - one pass of rows) rather than max_i (K_i + M_i);
+//   * each wave owns row blocks of the weights and streams their packed fragments
+//     as one sequence, PF k-steps per chunk, the next chunk in flight across row
+//     passes and GEMM boundaries; B fragments are read one step ahead;
 //   * INT8 uses u8(a).b = s8(a ^ 0x80).b + 128 sum_k b: the column sums of each X_i
 //     are taken from LDS and seed the accumulators;
 //   * only the last GEMM writes HBM (C, fp32 or int32, row-major).
@@ -33,28 +36,28 @@ constexpr size_t LDS_LIMIT = 160 * 1024;
 
 template <int KIND> struct CT;
 // KG: k per 16-byte LDS group (per plane for F32X3); KC: k per MFMA step;
-// KA: A elements per lane-half per step; ESZ: global element bytes; LB: LDS bytes
-// per stored element (both planes for F32X3); RP: row blocks per wave item;
+// KA: A elements per lane-half per step; ESZ: global element bytes; RP: row blocks
+// per wave pass;
 // PF: k-steps per prefetch chunk; FS: packed bytes per lane per fragment.
 template <> struct CT<K_F32> {
-  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, LB = 4, RP = 1, PF = 4, FS = 16;
+  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, RP = 1, PF = 4, FS = 16;
   using Acc = f32x16;
 };
 template <> struct CT<K_BF16> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 2, LB = 2, RP = 2, PF = 4, FS = 16;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 2, RP = 2, PF = 4, FS = 16;
   using Acc = f32x16;
 };
 template <> struct CT<K_I8> {
-  static constexpr int KG = 16, KC = 32, KA = 16, ESZ = 1, LB = 1, RP = 2, PF = 4, FS = 16;
+  static constexpr int KG = 16, KC = 32, KA = 16, ESZ = 1, RP = 2, PF = 4, FS = 16;
   using Acc = i32x16;
 };
 template <> struct CT<K_F32X3> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, LB = 4, RP = 1, PF = 2, FS = 32;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, RP = 1, PF = 2, FS = 32;
   using Acc = f32x16;
 };
 
 template <int TN>
-__device__ __forceinline__ int cslot(int g, int n) { return g * TN + (n ^ (g & 7)); }
+__device__ __forceinline__ int cslot(int g, int n) { return g * TN + n; }
 
 // --------------------------------------------------------- packed weights --
 // Packed layout of A (M x K) for kind KIND: fragments f = (rg * kspad + ks) * RP + i
@@ -129,21 +132,11 @@ __global__ __launch_bounds__(256) void pack_a_kernel(PackJob J) {
 template <int KIND> struct AFrag { uint4 v[KIND == K_F32X3 ? 2 : 1]; };
 
 // One lane's B fragments for one k-step: TNB column blocks (hi and lo planes for
-// F32X3), read from the arena one step ahead of their MFMAs.
+// F32X3).
 template <int KIND, int TNB> struct BFrag {
   uint4 b[TNB];
   uint4 lo[KIND == K_F32X3 ? TNB : 1];
 };
-
-template <int KIND, int TNB>
-__device__ __forceinline__ void read_bfrag(const uint4 *arena, int lo_off, int g, int l32, BFrag<KIND, TNB> &f) {
-  constexpr int TN = TNB * 32;
-#pragma unroll
-  for (int j = 0; j < TNB; ++j) {
-    f.b[j] = arena[cslot<TN>(g, j * 32 + l32)];
-    if (KIND == K_F32X3) f.lo[j] = arena[lo_off + cslot<TN>(g, j * 32 + l32)];
-  }
-}
 
 template <int KIND, int TNB>
 __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KIND> (&af)[CT<KIND>::RP],
@@ -240,9 +233,10 @@ __device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K
   }
 }
 
-// INT8: colsum[n] = sum over rows [xbase, xbase + Kp) of s8 X[k][n].
+// INT8: colsum[n] = sum over the ngroups k-groups of X starting at group g0 of the
+// circular arena (SG groups) of s8 X[k][n].
 template <int TN>
-__device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int *colsum) {
+__device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int SG, int *colsum) {
   // thread -> (column, slice of groups); reduce slices with LDS atomics (integer adds
   // are order independent, so the result is deterministic).
   for (int n = threadIdx.x; n < TN; n += NTH) colsum[n] = 0;
@@ -252,7 +246,8 @@ __device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int *colsum)
   if (sl < SL) {
     int s = 0;
     for (int g = sl; g < ngroups; g += SL) {
-      uint4 x = arena[cslot<TN>(g0 + g, n)];
+      const int ga = g0 + g < SG ? g0 + g : g0 + g - SG;
+      uint4 x = arena[cslot<TN>(ga, n)];
       s = __builtin_amdgcn_sdot4((int)x.x, 0x01010101, s, false);
       s = __builtin_amdgcn_sdot4((int)x.y, 0x01010101, s, false);
       s = __builtin_amdgcn_sdot4((int)x.z, 0x01010101, s, false);
@@ -262,25 +257,147 @@ __device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int *colsum)
   }
 }
 
+// Epilogue of one pass: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32 of
+// row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena.
+template <int KIND, int TNB>
+__device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, long rb0, long n0, uint4 *arena,
+                                                const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
+  using T = CT<KIND>;
+  constexpr int TN = TNB * 32;
+  constexpr int RP = T::RP;
+  const int SG = P.S / T::KG;
+  const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;
+  const int lane = threadIdx.x & 63;
+  const int h = lane >> 5, l32 = lane & 31;
+  const long M = P.M[gi], Mp = (M + 31) / 32 * 32;
+  const bool last = gi == P.num_gemm - 1;
+  const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
+#pragma unroll
+  for (int j = 0; j < TNB; ++j) {
+    const int n = j * 32 + l32;
+#pragma unroll
+    for (int i = 0; i < RP; ++i) {
+      const long rbase = rb0 + i * 32;
+      if (rbase >= Mp) continue;
+      if (last) {
+        if (n >= tile_w) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          long m = rbase + 8 * (r >> 2) + 4 * h + (r & 3);
+          if (m >= M) continue;
+          size_t off = (size_t)m * P.N + n0 + n;
+          if (KIND == K_I8) ((int *)P.C)[off] = (int)acc[i][j][r];
+          else ((float *)P.C)[off] = (float)acc[i][j][r];
+        }
+      } else {
+        // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q.
+        // Padded rows (m >= M) are stored as zero: they are the next GEMM's k-padding
+        // (and INT8 column sums include them).
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const long m0r = rbase + 8 * q + 4 * h;
+          long row = P.ybase[gi] + m0r;  // arena row, multiple of 4
+          if (row >= P.S) row -= P.S;
+          const int g = (int)(row / T::KG), pos = (int)(row % T::KG);
+          uint4 *dst = arena + cslot<TN>(g, n);
+          if (KIND == K_I8) {
+            uint32_t w = sat_pk_i8((int)acc[i][j][4 * q], (int)acc[i][j][4 * q + 1], (int)acc[i][j][4 * q + 2],
+                                   (int)acc[i][j][4 * q + 3]);
+            if (m0r + 4 > M) w &= m0r >= M ? 0u : (0xffffffffu >> (8 * (m0r + 4 - M)));
+            *((uint32_t *)dst + (pos >> 2)) = w;
+          } else {
+            float v[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = m0r + e < M ? (float)acc[i][j][4 * q + e] : 0.0f;
+            if (KIND == K_F32) {
+              *dst = make_uint4(__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
+                                __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3]));
+            } else if (KIND == K_BF16) {
+              *((uint2 *)dst + (pos >> 2)) = make_uint2(cvt_pk_bf16(v[0], v[1]), cvt_pk_bf16(v[2], v[3]));
+            } else {
+              uint32_t h01, l01, h23, l23;
+              split_pk(v[0], v[1], h01, l01);
+              split_pk(v[2], v[3], h23, l23);
+              *((uint2 *)dst + (pos >> 2)) = make_uint2(h01, h23);
+              *((uint2 *)(arena + lo_off + cslot<TN>(g, n)) + (pos >> 2)) = make_uint2(l01, l23);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
 // One workgroup runs the whole chain on columns [n0, n0 + TN) of B. P.A[i] are
 // packed weights (pack_a_kernel layout for KIND).
+//
+// The arena is circular (P.S rows). X_i starts at xbase[i] and spans K_i rounded up
+// to a chunk (PF k-steps, the k-padding of the packed weights), Y_i follows it at
+// ybase[i] (mod S); regions start on chunk boundaries, so no chunk of X wraps and
+// every B read of a chunk is the chunk's base plus an immediate offset. Rows of X
+// past K are zero. Each wave keeps its final row pass of every GEMM in registers
+// until all waves are done with X_i, then writes it, possibly over X_i; earlier
+// passes go to rows that never meet X_i. Per wave, weights and B are double
+// buffered a whole chunk ahead: while chunk c's MFMAs run, chunk c+1's weight
+// loads (global) and B reads (LDS) are in flight. The weight stream continues
+// across row passes and GEMM boundaries.
 template <int KIND, int TNB>
-__device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
+__device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
   constexpr int TN = TNB * 32;
   constexpr int RP = T::RP;
   constexpr int PF = T::PF;
-  constexpr int FRAG = 64 * T::FS;  // bytes per packed fragment
-  const int lo_off = (KIND == K_F32X3) ? (P.S / 8) * TN : 0;  // lo plane (F32X3)
-  int *colsum = (int *)(arena + (size_t)(P.S / T::KG) * TN * (KIND == K_F32X3 ? 2 : 1));
+  constexpr int FRAG = 64 * T::FS;           // bytes per packed fragment
+  constexpr long PASS = (long)NW * RP * 32;  // rows of one pass of all waves
+  constexpr long CS = (long)PF * T::KC;      // rows of one chunk
+  const int SG = P.S / T::KG;                // arena k-groups
+  const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;  // lo plane (F32X3), in slots
+  int *colsum = (int *)(arena + (size_t)SG * TN * (KIND == K_F32X3 ? 2 : 1));
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
   const int h = lane >> 5, l32 = lane & 31;
-  const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
 
-  const long K1p = (P.K[0] + 31) / 32 * 32;
-  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1p, P.N, n0);
+  // first packed-weight chunk of this wave in GEMM gi (nullptr: no rows there);
+  // fragment (ks, i) of a pass at base + (ks * RP + i) * FRAG
+  auto first_ptr = [&](int gi) -> const char * {
+    if ((long)wave * RP * 32 >= (P.M[gi] + 31) / 32 * 32) return nullptr;
+    return (const char *)P.A[gi] + (size_t)wave * kspad_of<KIND>(P.K[gi]) * RP * FRAG + lane * 16;
+  };
+  AFrag<KIND> fa[2][PF][RP];
+  auto load_chunk = [&](const char *base, AFrag<KIND>(&dst)[PF][RP]) {
+#pragma unroll
+    for (int s = 0; s < PF; ++s)
+#pragma unroll
+      for (int i = 0; i < RP; ++i) {
+        const char *p = base + (size_t)(s * RP + i) * FRAG;
+        dst[s][i].v[0] = *(const uint4 *)p;
+        if (KIND == K_F32X3) dst[s][i].v[1] = *(const uint4 *)(p + 1024);
+      }
+  };
+  constexpr size_t CHUNK = (size_t)PF * RP * FRAG;
+
+  // B fragments of one chunk: step s, column block j of this lane at slot
+  // (g0 + 2 s + h) * TN + 32 j + l32 for the chunk's first group g0
+  BFrag<KIND, TNB> fb[2][PF];
+  const uint4 *lane_b = arena + h * TN + l32;
+  auto read_chunk_b = [&](int g0, BFrag<KIND, TNB>(&dst)[PF]) {
+    const uint4 *p = lane_b + g0 * TN;
+#pragma unroll
+    for (int s = 0; s < PF; ++s)
+#pragma unroll
+      for (int j = 0; j < TNB; ++j) {
+        dst[s].b[j] = p[2 * s * TN + 32 * j];
+        if (KIND == K_F32X3) dst[s].lo[j] = p[lo_off + 2 * s * TN + 32 * j];
+      }
+  };
+
+  // invariant: at the start of each pass fa[0] holds (or has in flight) chunk 0
+  const char *apk = first_ptr(0);
+  if (apk) load_chunk(apk, fa[0]);
+  const long K1c = (P.K[0] + CS - 1) / CS * CS;
+  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0);
   __syncthreads();
 
   for (int gi = 0; gi < P.num_gemm; ++gi) {
@@ -288,120 +405,90 @@ __device__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
     const long Mp = (M + 31) / 32 * 32;
     const int xg0 = P.xbase[gi] / T::KG;
     const bool last = gi == P.num_gemm - 1;
-    if (KIND == K_I8) {
-      colsum_lds<TN>(arena, xg0, (int)((K + 31) / 32 * 32 / T::KG), colsum);
-      __syncthreads();
-    }
-    // k-steps padded to whole chunks (the packed A is zero there); B reads past the
-    // X region clamp to its last group, which then meets zero A.
+    // k-steps padded to whole chunks: the packed weights are zero there and so are
+    // the rows of X
     const int kspad = (int)kspad_of<KIND>(K);
     const int nchunks = kspad / PF;
-    const int glast = xg0 + (int)((K + 31) / 32 * 32 / T::KG) - 1;
+    if (KIND == K_I8) {
+      colsum_lds<TN>(arena, xg0, (int)(kspad * T::KC / T::KG), SG, colsum);
+      __syncthreads();
+    }
+    const size_t pass_stride = (size_t)NW * kspad * RP * FRAG;
+    const char *next_gemm = last ? nullptr : first_ptr(gi + 1);
+    auto chunk_g0 = [&](int c) {  // first arena group of chunk c of X_gi
+      const int g = xg0 + 2 * PF * c;
+      return g < SG ? g : g - SG;
+    };
     // INT8: the accumulators start at the bias 128 * sum_k X[k][n] of their column,
     // so the MFMAs accumulate u8(a).b directly.
     int bias0[TNB];
 #pragma unroll
     for (int j = 0; j < TNB; ++j) bias0[j] = (KIND == K_I8) ? 128 * colsum[j * 32 + l32] : 0;
-    for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += (long)NW * RP * 32) {
-      Acc acc[RP][TNB];
+
+    Acc acc[RP][TNB];
+    long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
+    if ((long)wave * RP * 32 < Mp) read_chunk_b(chunk_g0(0), fb[0]);
+    for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += PASS) {
 #pragma unroll
       for (int i = 0; i < RP; ++i)
 #pragma unroll
         for (int j = 0; j < TNB; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r) acc[i][j][r] = bias0[j];
+      const bool final_pass = rb0 + PASS >= Mp;
+      const char *nxt = final_pass ? next_gemm : apk + pass_stride;
+      // With no next pass the prefetch re-reads this pass's first chunk: the loads
+      // stay unconditional, so their wait counts are exact.
+      const char *after = nxt ? nxt : apk;
 
-      // this wave's row group in the packed weights; fragment (ks, i) at
-      // apk + (ks * RP + i) * FRAG, this lane's 16 bytes at + lane * 16
-      const char *apk = (const char *)P.A[gi] + ((size_t)(rb0 / (32 * RP)) * kspad * RP) * FRAG + lane * 16;
-      AFrag<KIND> fa[2][PF][RP];
-      auto load_chunk = [&](int c, AFrag<KIND>(&dst)[PF][RP]) {
+      auto mma_chunk = [&](const AFrag<KIND>(&a)[PF][RP], const BFrag<KIND, TNB>(&b)[PF]) {
 #pragma unroll
-        for (int s = 0; s < PF; ++s)
-#pragma unroll
-          for (int i = 0; i < RP; ++i) {
-            const char *p = apk + ((size_t)(c * PF + s) * RP + i) * FRAG;
-            dst[s][i].v[0] = *(const uint4 *)p;
-            if (KIND == K_F32X3) dst[s][i].v[1] = *(const uint4 *)(p + 1024);
-          }
+        for (int s = 0; s < PF; ++s) mma_k<KIND, TNB>(b[s], a[s], acc);
       };
-      // B fragments one step ahead, double buffered (PF is even, so the buffer of
-      // step ks is ks & 1 across chunks).
-      BFrag<KIND, TNB> bq[2];
-      auto run_chunk = [&](int c, const AFrag<KIND>(&src)[PF][RP]) {
-#pragma unroll
-        for (int s = 0; s < PF; ++s) {
-          const int ks = c * PF + s;
-          const int gn = xg0 + 2 * (ks + 1) + h;
-          read_bfrag<KIND, TNB>(arena, lo_off, gn < glast ? gn : glast, l32, bq[(s + 1) & 1]);
-          mma_k<KIND, TNB>(bq[s & 1], src[s], acc);
-        }
-      };
-      read_bfrag<KIND, TNB>(arena, lo_off, (xg0 + h) < glast ? xg0 + h : glast, l32, bq[0]);
-      load_chunk(0, fa[0]);
-      for (int c = 0; c < nchunks; c += 2) {
-        if (c + 1 < nchunks) load_chunk(c + 1, fa[1]);
-        run_chunk(c, fa[0]);
-        if (c + 1 < nchunks) {
-          if (c + 2 < nchunks) load_chunk(c + 2, fa[0]);
-          run_chunk(c + 1, fa[1]);
-        }
+      // The scheduling fences keep each chunk's loads and B reads ahead of the
+      // previous chunk's MFMAs.
+      int c = 0;
+      for (; c + 1 < nchunks; c += 2) {
+        load_chunk(apk + (c + 1) * CHUNK, fa[1]);
+        read_chunk_b(chunk_g0(c + 1), fb[1]);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_chunk(fa[0], fb[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        const bool more = c + 2 < nchunks;
+        load_chunk(more ? apk + (c + 2) * CHUNK : after, fa[0]);
+        read_chunk_b(chunk_g0(more ? c + 2 : 0), fb[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        mma_chunk(fa[1], fb[1]);
+        __builtin_amdgcn_sched_barrier(0);
       }
-
-      // epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32
-#pragma unroll
-      for (int j = 0; j < TNB; ++j) {
-        const int n = j * 32 + l32;
-#pragma unroll
-        for (int i = 0; i < RP; ++i) {
-          const long rbase = rb0 + i * 32;
-          if (rbase >= Mp) continue;
-          if (last) {
-            if (n >= tile_w) continue;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-              long m = rbase + 8 * (r >> 2) + 4 * h + (r & 3);
-              if (m >= M) continue;
-              size_t off = (size_t)m * P.N + n0 + n;
-              if (KIND == K_I8) ((int *)P.C)[off] = (int)acc[i][j][r];
-              else ((float *)P.C)[off] = (float)acc[i][j][r];
-            }
-          } else {
-            // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              const long row = P.ybase[gi] + rbase + 8 * q + 4 * h;  // arena row, multiple of 4
-              const int g = (int)(row / T::KG), pos = (int)(row % T::KG);
-              uint4 *dst = arena + cslot<TN>(g, n);
-              if (KIND == K_F32) {
-                *dst = make_uint4(__builtin_bit_cast(uint32_t, (float)acc[i][j][4 * q]),
-                                  __builtin_bit_cast(uint32_t, (float)acc[i][j][4 * q + 1]),
-                                  __builtin_bit_cast(uint32_t, (float)acc[i][j][4 * q + 2]),
-                                  __builtin_bit_cast(uint32_t, (float)acc[i][j][4 * q + 3]));
-              } else if (KIND == K_BF16) {
-                *((uint2 *)dst + (pos >> 2)) =
-                    make_uint2(cvt_pk_bf16(acc[i][j][4 * q], acc[i][j][4 * q + 1]),
-                               cvt_pk_bf16(acc[i][j][4 * q + 2], acc[i][j][4 * q + 3]));
-              } else if (KIND == K_I8) {
-                // padded rows (m >= M) must stay zero: they enter the next column sum
-                const long m0r = rbase + 8 * q + 4 * h;
-                uint32_t w = sat_pk_i8((int)acc[i][j][4 * q], (int)acc[i][j][4 * q + 1], (int)acc[i][j][4 * q + 2],
-                                       (int)acc[i][j][4 * q + 3]);
-                if (m0r + 4 > M) w &= m0r >= M ? 0u : (0xffffffffu >> (8 * (m0r + 4 - M)));
-                *((uint32_t *)dst + (pos >> 2)) = w;
-              } else {
-                uint32_t h01, l01, h23, l23;
-                split_pk(acc[i][j][4 * q], acc[i][j][4 * q + 1], h01, l01);
-                split_pk(acc[i][j][4 * q + 2], acc[i][j][4 * q + 3], h23, l23);
-                *((uint2 *)dst + (pos >> 2)) = make_uint2(h01, h23);
-                *((uint2 *)(arena + lo_off + cslot<TN>(g, n)) + (pos >> 2)) = make_uint2(l01, l23);
-              }
-            }
-          }
-        }
+      if (c < nchunks) {  // odd chunk count: the next loads wait for this chunk
+        mma_chunk(fa[0], fb[0]);
+        __builtin_amdgcn_sched_barrier(0);
+        load_chunk(after, fa[0]);
+        read_chunk_b(chunk_g0(0), fb[0]);
+      }
+      apk = after;
+      if (final_pass && !last) held_rb0 = rb0;
+      else store_rows_impl<KIND, TNB>(P, gi, rb0, n0, arena, acc);
+    }
+    if ((long)wave * RP * 32 >= Mp && next_gemm) {  // no rows here: fetch the next GEMM's first chunk
+      apk = next_gemm;
+      load_chunk(apk, fa[0]);
+    }
+    __syncthreads();  // every wave is done reading X_gi
+    if (last) break;
+    if (held_rb0 >= 0) store_rows_impl<KIND, TNB>(P, gi, held_rb0, n0, arena, acc);
+    // rows [Mp, Mc) of Y_gi (the k-padding of the next GEMM) must read as zero
+    {
+      const long Mc = kspad_of<KIND>(M) * T::KC;
+      const int zg0 = (int)((P.ybase[gi] + Mp) / T::KG), nzg = (int)((Mc - Mp) / T::KG);
+      for (int t = threadIdx.x; t < nzg * TN; t += NTH) {
+        const int g = zg0 + t / TN < SG ? zg0 + t / TN : zg0 + t / TN - SG;
+        arena[cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
+        if (KIND == K_F32X3) arena[lo_off + cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
       }
     }
-    __syncthreads();
+    __syncthreads();  // Y_gi complete
   }
 }
 
@@ -411,37 +498,64 @@ __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   chain_tile<KIND, TNB>(P, (long)blockIdx.x * (TNB * 32), arena);
 }
 
-constexpr int LB_of(int kind) { return kind == K_F32 ? 4 : kind == K_BF16 ? 2 : kind == K_I8 ? 1 : 4; }
+constexpr int RP_of(int kind) {
+  return kind == K_F32 ? CT<K_F32>::RP : kind == K_BF16 ? CT<K_BF16>::RP : kind == K_I8 ? CT<K_I8>::RP
+                                                                                        : CT<K_F32X3>::RP;
+}
+constexpr int KG_of(int kind) {
+  return kind == K_F32 ? CT<K_F32>::KG : kind == K_BF16 ? CT<K_BF16>::KG : kind == K_I8 ? CT<K_I8>::KG
+                                                                                        : CT<K_F32X3>::KG;
+}
+// rows of one chunk (PF k-steps): the granule of X regions and of the weights' k-padding
+constexpr int CS_of(int kind) {
+  return kind == K_F32 ? CT<K_F32>::PF * CT<K_F32>::KC
+         : kind == K_BF16 ? CT<K_BF16>::PF * CT<K_BF16>::KC
+         : kind == K_I8   ? CT<K_I8>::PF * CT<K_I8>::KC
+                          : CT<K_F32X3>::PF * CT<K_F32X3>::KC;
+}
 
-// Arena rows and region placement. X_0 (the B tile) sits at row 0; GEMM i writes
-// Y_i at the end opposite to X_i.
-int plan_arena(const long *M, const long *K, int n, int *xbase, int *ybase) {
-  auto r32 = [](long v) { return (int)((v + 31) / 32 * 32); };
-  int S = r32(K[0]);
+// Circular arena of S rows: X_0 (the B tile) at row 0, Y_i right after X_i (mod S)
+// and X_{i+1} = Y_i; every region spans its k-extent rounded up to a chunk and starts
+// on a chunk boundary (S is a multiple of a chunk). All rows of Y_i but each wave's
+// final pass are written while X_i is still read, so they must not meet it:
+// Kc_i + (passes_i - 1) * PASS <= S.
+int plan_arena(int kind, const long *M, const long *K, int n, int *xbase, int *ybase) {
+  const long cs = CS_of(kind);
+  auto rc = [cs](long v) { return (v + cs - 1) / cs * cs; };
+  auto r32 = [](long v) { return (v + 31) / 32 * 32; };
+  const long pass = (long)NW * RP_of(kind) * 32;
+  long S = rc(K[0]);
   for (int i = 0; i < n; ++i) {
-    int kx = (i == 0) ? r32(K[0]) : r32(M[i - 1]);
-    if (kx > S) S = kx;
-    if (i < n - 1 && kx + r32(M[i]) > S) S = kx + r32(M[i]);
+    const long kc = rc(K[i]);
+    S = std::max(S, kc);
+    if (i < n - 1) {
+      const long passes = (r32(M[i]) + pass - 1) / pass;
+      S = std::max(S, std::max(rc(M[i]), kc + (passes - 1) * pass));
+    }
   }
-  bool at_start = true;  // X_0 at the start
+  S = rc(S);
+  long x = 0;
   for (int i = 0; i < n; ++i) {
-    int kx = (i == 0) ? r32(K[0]) : r32(M[i - 1]);
-    xbase[i] = at_start ? 0 : S - kx;
-    if (i < n - 1) ybase[i] = at_start ? S - r32(M[i]) : 0;
-    else ybase[i] = 0;
-    at_start = !at_start;
+    xbase[i] = (int)x;
+    ybase[i] = (int)((x + rc(K[i])) % S);
+    x = ybase[i];
   }
-  return S;
+  return (int)S;
+}
+
+// LDS bytes: S rows (both planes for F32X3) and the INT8 column sums.
+size_t arena_bytes(int kind, int S, int tile_n) {
+  const size_t groups = (size_t)S / KG_of(kind);
+  return groups * tile_n * 16 * (kind == K_F32X3 ? 2 : 1) + (kind == K_I8 ? tile_n * sizeof(int) : 0);
 }
 
 template <int KIND, int TNB>
 hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
-  using T = CT<KIND>;
   ChainParams P{};
   make_chain_params(c, &P);
   if (!chain_fused_aligned(P)) return hipErrorInvalidValue;
   constexpr int TN = TNB * 32;
-  size_t lds = (size_t)P.S * TN * T::LB + (KIND == K_I8 ? TN * sizeof(int) : 0);
+  const size_t lds = arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
   auto kfn = chain_kernel<KIND, TNB>;
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -452,14 +566,7 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
 }
 
 // Tile widths of the grouped kernel, one per kind.
-constexpr int GT_F32 = 1, GT_BF16 = 1, GT_I8 = 2, GT_F32X3 = 1;
-
-// Out-of-line tile bodies for the grouped kernel: each kind keeps its own register
-// allocation instead of one allocation for the union of all four.
-template <int KIND, int TNB>
-__device__ __noinline__ void chain_tile_call(const ChainParams &P, long n0, uint4 *arena) {
-  chain_tile<KIND, TNB>(P, n0, arena);
-}
+constexpr int GT_F32 = 1, GT_BF16 = 1, GT_I8 = 1, GT_F32X3 = 1;
 
 // Persistent grouped launch over a packed table (GroupTable header, the request
 // descriptors, the tile list): workgroup b runs tiles b, b + grid, ...; each tile is
@@ -475,22 +582,30 @@ __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restri
     const int2 w = items[it];
     const ChainParams &P = reqs[w.x];
     switch (P.kind) {
-      case K_F32: chain_tile_call<K_F32, GT_F32>(P, (long)w.y * GT_F32 * 32, arena); break;
-      case K_BF16: chain_tile_call<K_BF16, GT_BF16>(P, (long)w.y * GT_BF16 * 32, arena); break;
-      case K_I8: chain_tile_call<K_I8, GT_I8>(P, (long)w.y * GT_I8 * 32, arena); break;
-      case K_F32X3: chain_tile_call<K_F32X3, GT_F32X3>(P, (long)w.y * GT_F32X3 * 32, arena); break;
+      case K_F32: chain_tile<K_F32, GT_F32>(P, (long)w.y * GT_F32 * 32, arena); break;
+      case K_BF16: chain_tile<K_BF16, GT_BF16>(P, (long)w.y * GT_BF16 * 32, arena); break;
+      case K_I8: chain_tile<K_I8, GT_I8>(P, (long)w.y * GT_I8 * 32, arena); break;
+      case K_F32X3: chain_tile<K_F32X3, GT_F32X3>(P, (long)w.y * GT_F32X3 * 32, arena); break;
     }
     __syncthreads();  // the next tile restages the arena
   }
 }
+
+// Widest tile per kind: BF16 and INT8 keep two row blocks per wave, so wider tiles
+// would not fit their accumulators and double-buffered fragments in 256 VGPRs.
+constexpr int max_tile_of(int kind) { return (kind == K_BF16 || kind == K_I8) ? 64 : 128; }
 
 template <int KIND>
 hipError_t launch_chain_k(const ChainArgs &c, hipStream_t s) {
   switch (c.tile_n) {
     case 32: return launch_chain_t<KIND, 1>(c, s);
     case 64: return launch_chain_t<KIND, 2>(c, s);
-    case 96: return launch_chain_t<KIND, 3>(c, s);
-    case 128: return launch_chain_t<KIND, 4>(c, s);
+  }
+  if constexpr (max_tile_of(KIND) == 128) {
+    switch (c.tile_n) {
+      case 96: return launch_chain_t<KIND, 3>(c, s);
+      case 128: return launch_chain_t<KIND, 4>(c, s);
+    }
   }
   return hipErrorInvalidValue;
 }
@@ -510,11 +625,13 @@ hipError_t launch_pack_t(PackJob J, hipStream_t s) {
 size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n) {
   int xb[MAXG], yb[MAXG];
   if (num_gemm < 1 || num_gemm > MAXG) return (size_t)-1;
-  int S = plan_arena(M, K, num_gemm, xb, yb);
-  return (size_t)S * tile_n * LB_of(kind) + (kind == K_I8 ? tile_n * sizeof(int) : 0);
+  const int S = plan_arena(kind, M, K, num_gemm, xb, yb);
+  return arena_bytes(kind, S, tile_n);
 }
 
 int chain_fused_max_gemms() { return MAXG; }
+
+int chain_fused_max_tile(int kind) { return max_tile_of(kind); }
 
 size_t packed_a_bytes(int kind, long M, long K) {
   switch (kind) {
@@ -562,7 +679,7 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
     P->M[i] = c.M[i];
     P->K[i] = c.K[i];
   }
-  P->S = plan_arena(c.M, c.K, c.num_gemm, P->xbase, P->ybase);
+  P->S = plan_arena(c.kind, c.M, c.K, c.num_gemm, P->xbase, P->ybase);
   P->B = c.B;
   P->C = c.C;
   P->N = c.N;

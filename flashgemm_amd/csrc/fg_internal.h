@@ -204,6 +204,7 @@ hipError_t launch_pack_a(int kind, const PackJob &J, hipStream_t s);
 hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s);
 size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n);
 int chain_fused_max_gemms();
+int chain_fused_max_tile(int kind);  // widest fused tile_n for the kind
 constexpr size_t LDS_BYTES_MAX = 160 * 1024;
 
 // Grouped persistent launch. The device table is a GroupTable header followed by

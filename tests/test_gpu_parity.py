@@ -201,8 +201,23 @@ def test_int8_chain_bit_exact_math_layout(gpu, K1, N, Ms, fuse):
 def test_int8_chain_every_tile_width(gpu, tile_n):
     rng = np.random.default_rng(tile_n)
     B, As = rand_chain(rng, fg.DTYPE_INT8, 128, 300, [64, 96, 48], full_int8=True)
+    if tile_n > 64:  # INT8 fused tiles are at most 64 columns wide
+        with pytest.raises(fg.FlashgemmError) as e:
+            run_chain(fg.DTYPE_INT8, B, As, fuse=fg.FUSE_ON, tile_n=tile_n)
+        assert e.value.status == fg.ERR_UNSUPPORTED
+        return
     got = run_chain(fg.DTYPE_INT8, B, As, fuse=fg.FUSE_ON, tile_n=tile_n)
     assert np.array_equal(got, O.chain_int8(B, As, O.LAYOUT_MATH))
+
+
+@pytest.mark.parametrize("f32_mode", [fg.F32_MFMA, fg.F32_BF16X3])
+@pytest.mark.parametrize("tile_n", [32, 64, 96, 128])
+def test_f32_chain_every_tile_width(gpu, tile_n, f32_mode):
+    rng = np.random.default_rng(tile_n + 7)
+    B, As = rand_chain(rng, fg.DTYPE_F32, 100, 300, [72, 40, 136])
+    got = run_chain(fg.DTYPE_F32, B, As, fuse=fg.FUSE_ON, tile_n=tile_n, f32_mode=f32_mode)
+    tol = TOL_F32 if f32_mode == fg.F32_MFMA else TOL_X3
+    assert O.rel_fro(got, O.chain_f32(B, As)) <= tol
 
 
 @pytest.mark.parametrize("K1,N,Ms", CHAIN_SHAPES)

@@ -259,68 +259,84 @@ __device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int SG, int 
 
 // Epilogue of one pass: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32 of
 // row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena.
+// Index math is 32-bit (arena rows and M fit easily) and the row-bound checks are
+// only evaluated in the block that straddles M.
 template <int KIND, int TNB>
-__device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, long rb0, long n0, uint4 *arena,
+__device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, long rb0_l, long n0, uint4 *arena,
                                                 const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
   using T = CT<KIND>;
   constexpr int TN = TNB * 32;
   constexpr int RP = T::RP;
-  const int SG = P.S / T::KG;
+  const int S = P.S;
+  const int SG = S / T::KG;
   const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;
   const int lane = threadIdx.x & 63;
   const int h = lane >> 5, l32 = lane & 31;
-  const long M = P.M[gi], Mp = (M + 31) / 32 * 32;
-  const bool last = gi == P.num_gemm - 1;
-  const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
-#pragma unroll
-  for (int j = 0; j < TNB; ++j) {
-    const int n = j * 32 + l32;
+  const int M = (int)P.M[gi], Mp = (M + 31) & ~31;
+  const int rb0 = (int)rb0_l;
+  if (gi == P.num_gemm - 1) {
+    const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
+    const size_t rs = (size_t)P.N * 4;  // bytes per row of C
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
-      const long rbase = rb0 + i * 32;
+      const int rbase = rb0 + i * 32;
       if (rbase >= Mp) continue;
-      if (last) {
+      const bool full = rbase + 32 <= M;
+#pragma unroll
+      for (int j = 0; j < TNB; ++j) {
+        const int n = j * 32 + l32;
         if (n >= tile_w) continue;
+        char *base = (char *)P.C + ((size_t)(rbase + 4 * h) * P.N + n0 + n) * 4;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          long m = rbase + 8 * (r >> 2) + 4 * h + (r & 3);
-          if (m >= M) continue;
-          size_t off = (size_t)m * P.N + n0 + n;
-          if (KIND == K_I8) ((int *)P.C)[off] = (int)acc[i][j][r];
-          else ((float *)P.C)[off] = (float)acc[i][j][r];
+          const int ro = 8 * (r >> 2) + (r & 3);
+          if (full || rbase + 4 * h + ro < M) {
+            if (KIND == K_I8) *(int *)(base + ro * rs) = (int)acc[i][j][r];
+            else *(float *)(base + ro * rs) = (float)acc[i][j][r];
+          }
         }
-      } else {
-        // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q.
-        // Padded rows (m >= M) are stored as zero: they are the next GEMM's k-padding
-        // (and INT8 column sums include them).
+      }
+    }
+    return;
+  }
+  // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q. Padded rows
+  // (m >= M) are stored as zero: they are the next GEMM's k-padding (and INT8 column
+  // sums include them).
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const long m0r = rbase + 8 * q + 4 * h;
-          long row = P.ybase[gi] + m0r;  // arena row, multiple of 4
-          if (row >= P.S) row -= P.S;
-          const int g = (int)(row / T::KG), pos = (int)(row % T::KG);
-          uint4 *dst = arena + cslot<TN>(g, n);
-          if (KIND == K_I8) {
-            uint32_t w = sat_pk_i8((int)acc[i][j][4 * q], (int)acc[i][j][4 * q + 1], (int)acc[i][j][4 * q + 2],
-                                   (int)acc[i][j][4 * q + 3]);
-            if (m0r + 4 > M) w &= m0r >= M ? 0u : (0xffffffffu >> (8 * (m0r + 4 - M)));
-            *((uint32_t *)dst + (pos >> 2)) = w;
+  for (int i = 0; i < RP; ++i) {
+    const int rbase = rb0 + i * 32;
+    if (rbase >= Mp) continue;
+    const bool full = rbase + 32 <= M;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int m0r = rbase + 8 * q + 4 * h;
+      int row = P.ybase[gi] + m0r;  // arena row, multiple of 4
+      if (row >= S) row -= S;
+      const int g = row / T::KG, pos = row % T::KG;
+      const int valid = full ? 4 : (M - m0r < 0 ? 0 : (M - m0r > 4 ? 4 : M - m0r));
+#pragma unroll
+      for (int j = 0; j < TNB; ++j) {
+        uint4 *dst = arena + cslot<TN>(g, j * 32 + l32);
+        if (KIND == K_I8) {
+          uint32_t w = sat_pk_i8((int)acc[i][j][4 * q], (int)acc[i][j][4 * q + 1], (int)acc[i][j][4 * q + 2],
+                                 (int)acc[i][j][4 * q + 3]);
+          if (!full) w &= valid == 0 ? 0u : (0xffffffffu >> (8 * (4 - valid)));
+          *((uint32_t *)dst + (pos >> 2)) = w;
+        } else {
+          float v[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[e] = (full || e < valid) ? (float)acc[i][j][4 * q + e] : 0.0f;
+          if (KIND == K_F32) {
+            *dst = make_uint4(__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
+                              __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3]));
+          } else if (KIND == K_BF16) {
+            *((uint2 *)dst + (pos >> 2)) = make_uint2(cvt_pk_bf16(v[0], v[1]), cvt_pk_bf16(v[2], v[3]));
           } else {
-            float v[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) v[e] = m0r + e < M ? (float)acc[i][j][4 * q + e] : 0.0f;
-            if (KIND == K_F32) {
-              *dst = make_uint4(__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
-                                __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3]));
-            } else if (KIND == K_BF16) {
-              *((uint2 *)dst + (pos >> 2)) = make_uint2(cvt_pk_bf16(v[0], v[1]), cvt_pk_bf16(v[2], v[3]));
-            } else {
-              uint32_t h01, l01, h23, l23;
-              split_pk(v[0], v[1], h01, l01);
-              split_pk(v[2], v[3], h23, l23);
-              *((uint2 *)dst + (pos >> 2)) = make_uint2(h01, h23);
-              *((uint2 *)(arena + lo_off + cslot<TN>(g, n)) + (pos >> 2)) = make_uint2(l01, l23);
-            }
+            uint32_t h01, l01, h23, l23;
+            split_pk(v[0], v[1], h01, l01);
+            split_pk(v[2], v[3], h23, l23);
+            *((uint2 *)dst + (pos >> 2)) = make_uint2(h01, h23);
+            *((uint2 *)(arena + lo_off + cslot<TN>(g, j * 32 + l32)) + (pos >> 2)) = make_uint2(l01, l23);
           }
         }
       }

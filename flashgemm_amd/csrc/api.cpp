@@ -281,6 +281,18 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
     if ((rc = scratch(s, "colsum", sizeof(int) * N, (void **)&colsum))) return rc;
     if ((rc = hip_status(launch_colsum_i8((const int8_t *)B, K, N, colsum, s), "colsum"))) return rc;
   }
+  if (kind == K_F32 && epi != EPI_PARTIAL && splits <= 0 && skinny_f32_shape(M, N)) {
+    // small M or N: 64 x 64 tiles, split-K over the grid
+    long kper = 0;
+    const int sk = skinny_f32_splits(M, N, K, &kper);
+    float *out = (float *)C;
+    if (sk > 1 && (rc = scratch(s, "partials", (size_t)sk * M * N * 4, (void **)&out))) return rc;
+    if ((rc = hip_status(launch_skinny_f32((const float *)A, (const float *)B, out, M, N, K, kper, sk, s),
+                         "skinny gemm")))
+      return rc;
+    if (sk == 1) return FG_OK;
+    return hip_status(launch_splitk_reduce(kind, EPI_STORE, out, sk, M, N, nullptr, C, s), "split-K reduce");
+  }
   if (splits <= 0) splits = plan_splits(M, N, K, kind);
   if (splits <= 1) {
     GemmArgs g{kind, epi, A, B, C, colsum, M, N, K, 1};

@@ -92,15 +92,28 @@ __device__ __forceinline__ void pack_frag(const char *__restrict__ A, long M, lo
   uint32_t w[T::KA * T::ESZ / 4];
 #pragma unroll
   for (int d = 0; d < T::KA * T::ESZ / 4; ++d) w[d] = 0u;
-  if (row < M) {
-    const char *p = A + ((size_t)row * K + k0) * T::ESZ;
+  const char *src = A + ((size_t)row * K + k0) * T::ESZ;
+  if (row < M && k0 + T::KA <= K && ((uintptr_t)src & 15) == 0) {
+    // interior, aligned: the lane's KA elements are 16-byte loads
+#pragma unroll
+    for (int d = 0; d < T::KA * T::ESZ / 16; ++d) {
+      const uint4 v = *(const uint4 *)(src + 16 * d);
+      w[4 * d] = v.x;
+      w[4 * d + 1] = v.y;
+      w[4 * d + 2] = v.z;
+      w[4 * d + 3] = v.w;
+    }
+    if (KIND == K_I8)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) w[d] ^= 0x80808080u;  // u8 -> s8 bias
+  } else if (row < M) {
 #pragma unroll
     for (int e = 0; e < T::KA; ++e)
       if (k0 + e < K) {
         uint32_t x;
-        if (T::ESZ == 4) x = *(const uint32_t *)(p + 4 * e);
-        else if (T::ESZ == 2) x = *(const uint16_t *)(p + 2 * e);
-        else x = (uint32_t)(*(const uint8_t *)(p + e) ^ 0x80u);  // u8 -> s8 bias
+        if (T::ESZ == 4) x = *(const uint32_t *)(src + 4 * e);
+        else if (T::ESZ == 2) x = *(const uint16_t *)(src + 2 * e);
+        else x = (uint32_t)(*(const uint8_t *)(src + e) ^ 0x80u);  // u8 -> s8 bias
         w[(e * T::ESZ) >> 2] |= x << (((e * T::ESZ) & 3) * 8);
       }
   }

@@ -144,6 +144,12 @@ struct GemmArgs {
 };
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
+// FP32 GEMMs with M <= 64 or N <= 64: 64 x 64 tiles, split-K (kper k per slab, slabs
+// summed by launch_splitk_reduce when splits > 1).
+bool skinny_f32_shape(long M, long N);
+int skinny_f32_splits(long M, long N, long K, long *kper);
+hipError_t launch_skinny_f32(const float *A, const float *B, float *C, long M, long N, long K, long kper,
+                             int splits, hipStream_t s);
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s);
 hipError_t launch_splitk_reduce(int kind, int epi, const void *partials, int splits, long M, long N,
                                 const int *colsum, void *C, hipStream_t s);

@@ -15,6 +15,8 @@
 //     epilogue), and FP32 split into bf16 hi/lo planes with three products.
 //   * epilogues: store C, requantize into the next GEMM's operand (fp32 / bf16
 //     RNE / int8 saturate), or write a split-K partial slab.
+#include <type_traits>
+
 #include "fg_internal.h"
 
 namespace fg {
@@ -50,10 +52,6 @@ template <> struct KT<K_F32X3> {
 constexpr int TILE_U4 = 16384 / 16;
 
 __device__ __forceinline__ int slot(int g, int r) { return g * 128 + (r ^ (g & 7)); }
-
-__device__ __forceinline__ uint32_t pack_bf16x2(uint16_t lo, uint16_t hi) {
-  return (uint32_t)lo | ((uint32_t)hi << 16);
-}
 
 // ---------------------------------------------------------------- staging --
 template <int KIND>
@@ -419,21 +417,36 @@ __global__ void colsum_i8_kernel(const int8_t *__restrict__ B, long K, long N, i
 
 // Fixed-order split-K reduction: partial slabs are added in slab order 0..S-1, so
 // the result does not depend on which workgroup finished first.
-template <int KIND, int EPI>
-__global__ void splitk_reduce_kernel(const void *__restrict__ P, int splits, long M, long N,
-                                     const int *__restrict__ colsum, void *__restrict__ C) {
-  size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  size_t MN = (size_t)M * N;
-  if (e >= MN) return;
+// Split-K reduction: a workgroup owns RE consecutive elements; RL lanes per element
+// each sum the slabs z = l, l + RL, ... in order, and lane sums are combined in
+// lane order through LDS, so the result does not depend on scheduling.
+constexpr int RE = 64;
+template <int KIND, int EPI, int RL>
+__global__ __launch_bounds__(RE * RL) void splitk_reduce_kernel(const void *__restrict__ P, int splits, long M,
+                                                                 long N, const int *__restrict__ colsum,
+                                                                 void *__restrict__ C) {
+  using Sum = typename std::conditional<KIND == K_I8, unsigned, float>::type;
+  __shared__ Sum part[RL][RE];
+  const int x = threadIdx.x % RE, l = threadIdx.x / RE;
+  const size_t MN = (size_t)M * N;
+  const size_t e = (size_t)blockIdx.x * RE + x;
+  Sum acc = 0;
+  if (e < MN) {
+    if (KIND == K_I8)
+      for (int z = l; z < splits; z += RL) acc += (unsigned)((const int *)P)[z * MN + e];
+    else
+      for (int z = l; z < splits; z += RL) acc += ((const float *)P)[z * MN + e];
+  }
+  part[l][x] = acc;
+  __syncthreads();
+  if (l != 0 || e >= MN) return;
+  Sum s = part[0][x];
+  for (int q = 1; q < RL; ++q) s += part[q][x];
   if (KIND == K_I8) {
-    unsigned s = 0;
-    for (int z = 0; z < splits; ++z) s += (unsigned)((const int *)P)[z * MN + e];
     s += 128u * (unsigned)colsum[e % N];
     if (EPI == EPI_STORE) ((int *)C)[e] = (int)s;
     else ((int8_t *)C)[e] = sat_i8((int)s);
   } else {
-    float s = 0.f;
-    for (int z = 0; z < splits; ++z) s += ((const float *)P)[z * MN + e];
     if (KIND == K_BF16 && EPI == EPI_REQUANT) ((uint16_t *)C)[e] = bf16_dev(s);
     else ((float *)C)[e] = s;
   }
@@ -501,8 +514,21 @@ hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream
 hipError_t launch_splitk_reduce(int kind, int epi, const void *P, int splits, long M, long N,
                                 const int *colsum, void *C, hipStream_t s) {
   size_t MN = (size_t)M * N;
-  dim3 grid((unsigned)((MN + 255) / 256)), block(256);
-#define FG_RED(K_, E_) hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_>), grid, block, 0, s, P, splits, M, N, colsum, C)
+  dim3 grid((unsigned)((MN + RE - 1) / RE));
+  // lanes per element: enough to keep every slab read in flight, at most 16
+  const int rl = splits >= 16 ? 16 : splits >= 8 ? 8 : splits >= 4 ? 4 : splits >= 2 ? 2 : 1;
+#define FG_RED1(K_, E_, L_) \
+  hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_, L_>), grid, dim3(RE * L_), 0, s, P, splits, M, N, colsum, C)
+#define FG_RED(K_, E_)                          \
+  do {                                          \
+    switch (rl) {                               \
+      case 16: FG_RED1(K_, E_, 16); break;      \
+      case 8: FG_RED1(K_, E_, 8); break;        \
+      case 4: FG_RED1(K_, E_, 4); break;        \
+      case 2: FG_RED1(K_, E_, 2); break;        \
+      default: FG_RED1(K_, E_, 1); break;       \
+    }                                           \
+  } while (0)
   if (kind == K_I8) {
     if (epi == EPI_STORE) FG_RED(K_I8, EPI_STORE); else FG_RED(K_I8, EPI_REQUANT);
   } else if (kind == K_BF16) {
@@ -511,6 +537,7 @@ hipError_t launch_splitk_reduce(int kind, int epi, const void *P, int splits, lo
     FG_RED(K_F32, EPI_STORE);
   }
 #undef FG_RED
+#undef FG_RED1
   return hipGetLastError();
 }
 
@@ -523,6 +550,153 @@ hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, c
   else
     hipLaunchKernelGGL((ref_permute_kernel<uint16_t>), grid, block, 0, s, (const uint16_t *)Z, (uint16_t *)X, rows, p,
                        false);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ skinny FP32 --
+// FP32 GEMMs with a small M or N (at most one 64-wide tile across it) are mostly
+// k-traffic: a 128 x 128 tile would waste half or three quarters of its MFMAs, so
+// they get a 64 x 64 tile per workgroup, split-K over the grid, no LDS staging.
+// Each of the 4 waves owns the whole 64 x 64 tile for k-groups g = wave, wave + 4,
+// ... (32 k per group) and loads its operands straight into MFMA fragment
+// registers, one group ahead: A[m][k..k+3] as 16 bytes per lane, B[k + s][n] as
+// coalesced dwords. The four wave tiles are summed through LDS in wave order.
+namespace {
+
+constexpr int SK_TILE = 64;
+constexpr int SK_GROUP = 32;  // k per wave group (4 MFMA k-steps of 8)
+
+struct SkFrag {
+  f32x4 a[4][2];     // [k-step][row block]: k = 8t + 4h .. 8t + 4h + 3
+  float b[4][2][4];  // [k-step][column block][s]: B[8t + 4h + s][32j + l32]
+};
+
+__device__ __forceinline__ void sk_load(const float *__restrict__ A, const float *__restrict__ B, long M, long N,
+                                        long K, long ke, long m0, long n0, long k, bool interior, int h, int l32,
+                                        SkFrag &f) {
+  if (interior) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        f.a[t][i] = *(const f32x4 *)(A + (m0 + 32 * i + l32) * K + k + 8 * t + 4 * h);
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) f.b[t][j][s] = B[(k + 8 * t + 4 * h + s) * N + n0 + 32 * j + l32];
+    }
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const long m = m0 + 32 * i + l32;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const long kk = k + 8 * t + 4 * h + s;
+        f.a[t][i][s] = (m < M && kk < ke) ? A[m * K + kk] : 0.0f;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const long n = n0 + 32 * j + l32;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const long kk = k + 8 * t + 4 * h + s;
+        f.b[t][j][s] = (n < N && kk < ke) ? B[kk * N + n] : 0.0f;
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict__ A, const float *__restrict__ B,
+                                                         float *__restrict__ C, long M, long N, long K, long kper,
+                                                         int tiles_n, int partial) {
+  __shared__ float red[SK_TILE * SK_TILE];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  const long m0 = (long)(blockIdx.x / tiles_n) * SK_TILE, n0 = (long)(blockIdx.x % tiles_n) * SK_TILE;
+  const long kb = (long)blockIdx.y * kper;
+  const long ke = kb + kper < K ? kb + kper : K;
+  const bool rows_in = m0 + SK_TILE <= M && n0 + SK_TILE <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0;
+  f32x16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  SkFrag f[2];
+  long k = kb + (long)wave * SK_GROUP;
+  if (k < ke) sk_load(A, B, M, N, K, ke, m0, n0, k, rows_in && k + SK_GROUP <= ke, h, l32, f[0]);
+  auto mma = [&](const SkFrag &g) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int i = 0; i < 2; ++i)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(g.a[t][i][s], g.b[t][j][s], acc[i][j], 0, 0, 0);
+  };
+  // two groups per iteration, buffers alternating
+  for (; k < ke; k += 8 * SK_GROUP) {
+    const long k1 = k + 4 * SK_GROUP, k2 = k + 8 * SK_GROUP;
+    if (k1 < ke) sk_load(A, B, M, N, K, ke, m0, n0, k1, rows_in && k1 + SK_GROUP <= ke, h, l32, f[1]);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(f[0]);
+    __builtin_amdgcn_sched_barrier(0);
+    if (k1 >= ke) break;
+    if (k2 < ke) sk_load(A, B, M, N, K, ke, m0, n0, k2, rows_in && k2 + SK_GROUP <= ke, h, l32, f[0]);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(f[1]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // sum the wave tiles in wave order: lane holds rows 8(r/4) + 4h + (r%4) of
+  // column 32j + l32 in row block i
+  for (int w = 0; w < 4; ++w) {
+    if (wave == w) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) {
+            const int row = 32 * i + 8 * (r >> 2) + 4 * h + (r & 3), col = 32 * j + l32;
+            if (w == 0) red[row * SK_TILE + col] = acc[i][j][r];
+            else red[row * SK_TILE + col] += acc[i][j][r];
+          }
+    }
+    __syncthreads();
+  }
+  float *out = partial ? C + (size_t)blockIdx.y * M * N : C;
+  for (int e = threadIdx.x; e < SK_TILE * SK_TILE; e += 256) {
+    const long m = m0 + e / SK_TILE, n = n0 + e % SK_TILE;
+    if (m < M && n < N) out[m * N + n] = red[e];
+  }
+}
+
+}  // namespace
+
+bool skinny_f32_shape(long M, long N) { return M <= SK_TILE || N <= SK_TILE; }
+
+int skinny_f32_splits(long M, long N, long K, long *kper_out) {
+  const long tiles = ((M + SK_TILE - 1) / SK_TILE) * ((N + SK_TILE - 1) / SK_TILE);
+  long want = (512 + tiles - 1) / tiles;  // about two workgroups per CU
+  long kper = (K + want - 1) / want;
+  kper = (kper + 4 * SK_GROUP - 1) / (4 * SK_GROUP) * (4 * SK_GROUP);
+  if (kper < 8 * SK_GROUP) kper = 8 * SK_GROUP;  // at least two groups per wave
+  *kper_out = kper;
+  return (int)((K + kper - 1) / kper);
+}
+
+hipError_t launch_skinny_f32(const float *A, const float *B, float *C, long M, long N, long K, long kper,
+                             int splits, hipStream_t s) {
+  const int tiles_m = (int)((M + SK_TILE - 1) / SK_TILE), tiles_n = (int)((N + SK_TILE - 1) / SK_TILE);
+  hipLaunchKernelGGL(skinny_f32_kernel, dim3((unsigned)(tiles_m * tiles_n), (unsigned)splits), dim3(256), 0, s, A, B,
+                     C, M, N, K, kper, tiles_n, splits > 1 ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -94,7 +94,6 @@ namespace {
 // One scheduler pass. Returns true if it did anything.
 bool step(fg_server *S) {
   bool did = false;
-  std::vector<Batch> finished;
   {
     std::lock_guard<std::mutex> lk(S->mu);
     // 1. retire completed batches
@@ -134,6 +133,9 @@ bool step(fg_server *S) {
   //    of this engine in flight, so a batch waits for a stream rather than queueing
   //    behind a long one)
   std::vector<long> batch;
+  // Descriptors of the batch, built under the lock: submit() and add_weights() may
+  // grow the request and weight-set vectors while this thread launches.
+  std::vector<fg_chain_req> q;
   int sidx = -1;
   hipEvent_t ev = nullptr;
   {
@@ -148,24 +150,7 @@ bool step(fg_server *S) {
       ev = S->free_events.back();
       S->free_events.pop_back();
     }
-  }
-  if (!ev) (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-  hipStream_t st = S->streams[sidx];
-  int rc = FG_OK;
-  if (S->cfg.per_gemm_baseline) {
-    for (long id : batch) {
-      const Request &r = S->reqs[id];
-      const WeightSet &w = S->sets[r.set];
-      fg_chain_opts o;
-      fg_chain_opts_default(&o);
-      o.f32_mode = w.f32_mode;
-      o.fuse = FG_FUSE_OFF;
-      o.splits = 1;
-      rc = fg_chain(w.dtype, r.C, r.B, r.N, w.num_gemm, (const void *const *)w.A, w.M, w.K, &o, st);
-      if (rc) break;
-    }
-  } else {
-    std::vector<fg_chain_req> q(batch.size()), fit, rest;
+    q.resize(batch.size());
     for (size_t i = 0; i < batch.size(); ++i) {
       const Request &r = S->reqs[batch[i]];
       const WeightSet &w = S->sets[r.set];
@@ -182,14 +167,28 @@ bool step(fg_server *S) {
         x.M[g] = w.M[g];
         x.K[g] = w.K[g];
       }
-      if (fg_grouped_fits(&x) == 1) {
+      if (!S->cfg.per_gemm_baseline && fg_grouped_fits(&x) == 1) {
         for (int g = 0; g < w.num_gemm; ++g) x.A[g] = w.packed[g];
         x.a_packed = 1;
-        fit.push_back(x);
-      } else {
-        rest.push_back(x);
       }
     }
+  }
+  if (!ev) (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  hipStream_t st = S->streams[sidx];
+  int rc = FG_OK;
+  if (S->cfg.per_gemm_baseline) {
+    for (const fg_chain_req &x : q) {
+      fg_chain_opts o;
+      fg_chain_opts_default(&o);
+      o.f32_mode = x.f32_mode;
+      o.fuse = FG_FUSE_OFF;
+      o.splits = 1;
+      rc = fg_chain(x.dtype, x.C, x.B, x.N, x.num_gemm, x.A, x.M, x.K, &o, st);
+      if (rc) break;
+    }
+  } else {
+    std::vector<fg_chain_req> fit, rest;
+    for (const fg_chain_req &x : q) (x.a_packed ? fit : rest).push_back(x);
     if (!fit.empty()) {
       if (S->cfg.use_graph)
         rc = fg_grouped_graph_launch(S->graphs[sidx], (int)fit.size(), fit.data(), st);

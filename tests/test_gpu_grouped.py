@@ -227,9 +227,9 @@ def test_server_per_gemm_baseline_matches(gpu):
             s = sets[i % 3]
             hB, dB, C = _request(rng, s, 100 + i)
             srv.submit(s["sid"], dB, C)
-            reqs.append((s, hB, C))
+            reqs.append((s, hB, dB, C))  # dB stays referenced until the request completes
         srv.drain()
-        for s, hB, C in reqs:
+        for s, hB, _, C in reqs:
             check_against_oracle(dict(dtype=s["dtype"], B=hB, As=s["As"]), C.cpu().numpy())
     finally:
         srv.close()

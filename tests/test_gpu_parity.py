@@ -68,7 +68,9 @@ def rand_chain(rng, dtype, K1, N, Ms, full_int8=False):
 
 # ------------------------------------------------------------ single GEMM --
 
-@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (128, 128, 256), (130, 70, 50), (1, 33, 17), (257, 129, 96)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 64), (128, 128, 256), (130, 70, 50), (1, 33, 17), (257, 129, 96),
+                                   # small M or N: the 64 x 64 split-K path, aligned and ragged
+                                   (1000, 64, 2000), (64, 300, 1024), (37, 45, 5001), (200, 8, 777)])
 def test_sgemm_fp32_mfma_vs_oracle(gpu, M, N, K):
     import torch
     print(f"sgemm {M}x{N}x{K}", flush=True)

@@ -14,8 +14,9 @@ Also reported (``configs``), each at its full size:
   * the FP32 precision study (4-chain, 16384 x 1024->4096->1024->4096->1024, columns
     scaled log-uniform in [1e-3, 1e3]): plain FP32 MFMA vs split-bf16 x3, relative
     Frobenius error of each against the FP64 oracle on the GPU;
-  * serving: p50/p99 latency of Poisson chain requests, continuous batching vs one
-    launch per GEMM (when the serving engine is built).
+  * serving: 10,000 Poisson chain requests against 64 resident weight sets, p50/p99
+    latency and throughput of the engine (continuous batching, grouped kernel under a
+    hipGraph) vs the same trace with one launch per GEMM.
 
 cpu_baseline: the reference itself (oracle/_ref, built from its sources) when this
 host has the AVX-512 features it needs, else the C restatement (oracle/), timed on
@@ -35,10 +36,10 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "int8": 5033.2, "fp64": 78.6}  # dense TFLOP/s (TOPS)
 HBM_GBS = 8000.0
-# HBM bytes per launch of the main chain kernel from rocprofv3 FETCH_SIZE + WRITE_SIZE,
-# each scaled by the same counter's reading on a device copy of known size
-# (bench.py --copy-calibration); None until measured.
-TRAFFIC_MAIN = None
+# HBM bytes per launch of the main chain kernel: rocprofv3 FETCH_SIZE 6196.5 KiB x 1.9997
+# + WRITE_SIZE 8192.0 KiB x 1.0000, the scales being what the same counters read on a
+# plain 256 MiB device copy (bench.py --copy-calibration). Algorithmic bytes: 17.30 MB.
+TRAFFIC_MAIN = 21077097
 
 
 def log(msg: str) -> None:
@@ -150,6 +151,19 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
         C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float32)
     fused, tn = fg.chain_plan(dtype, N, Ms, Ks)
     ms = timer.run(lambda: fg.chain(dtype, C, B, As, stream=stream), steps, warmup)
+    ms_kernel = None
+    if fused:  # the chain kernel alone: weights packed once (resident, as in serving)
+        packed = []
+        for a in As:
+            pk = torch.empty(fg.packed_weight_bytes(dtype, a.shape[0], a.shape[1]), device="cuda", dtype=torch.uint8)
+            fg.pack_weight(dtype, a, a.shape[0], a.shape[1], pk, stream=stream)
+            packed.append(pk)
+        Ck = torch.empty_like(C)
+        ms_kernel = timer.run(lambda: fg.chain(dtype, Ck, B, packed, N=N, Ms=Ms, Ks=Ks, o=fg.opts(a_packed=1),
+                                               stream=stream), steps, warmup)
+        torch.cuda.synchronize()
+        if not torch.equal(Ck, C):
+            raise RuntimeError("pre-packed chain differs from the drop-in call")
     Cu = torch.empty_like(C)
     ms_unfused = timer.run(lambda: fg.chain(dtype, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF), stream=stream),
                            steps, warmup)
@@ -158,9 +172,73 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
         float(torch.linalg.norm((C - Cu).double()) / torch.linalg.norm(Cu.double()))
     flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
     peak = PEAK["int8" if dtype == fg.DTYPE_INT8 else "bf16"]
-    return dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / peak, fused=fused, tile_n=tn,
-                ms_one_launch_per_gemm=ms_unfused, fused_vs_unfused=(same if dtype == fg.DTYPE_INT8 else None),
-                fused_vs_unfused_rel=(None if dtype == fg.DTYPE_INT8 else same))
+    out = dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / peak, fused=fused, tile_n=tn,
+               ms_one_launch_per_gemm=ms_unfused, fused_vs_unfused=(same if dtype == fg.DTYPE_INT8 else None),
+               fused_vs_unfused_rel=(None if dtype == fg.DTYPE_INT8 else same))
+    if ms_kernel is not None:
+        out.update(ms_kernel=ms_kernel, kernel_tflops=flop / ms_kernel / 1e9,
+                   kernel_frac_mfma=flop / ms_kernel / 1e9 / peak)
+    return out
+
+
+def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0), seed=5):
+    """10,000 chain requests with Poisson arrivals at a fixed rate against 64 resident
+    weight sets: chain length uniform in 2..4, widths from {64 .. 2048}, dtype mix 50%
+    BF16 / 30% INT8 / 20% FP32, request width (the reference's N, tokens) log-uniform
+    in 1..4096, HBM admission budget 200 GB. The same trace runs through the engine
+    (continuous batching, grouped persistent kernel replayed from a hipGraph) and
+    through the per-GEMM baseline (each request alone, one launch per GEMM)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    dims = [64, 128, 256, 512, 1024, 2048]
+    dts = rng.permutation([fg.DTYPE_BF16] * 32 + [fg.DTYPE_INT8] * 19 + [fg.DTYPE_F32] * 13)
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    sets = []
+    for dt in dts:
+        w = [int(rng.choice(dims)) for _ in range(int(rng.integers(2, 5)) + 1)]
+        As = []
+        for m, k in zip(w[1:], w[:-1]):
+            if dt == fg.DTYPE_INT8:
+                As.append(torch.randint(-127, 128, (m, k), generator=gen, device="cuda", dtype=torch.int8))
+            else:
+                a = torch.randn((m, k), generator=gen, device="cuda") / k ** 0.5
+                As.append(a.to(torch.bfloat16) if dt == fg.DTYPE_BF16 else a)
+        esz = 1 if dt == fg.DTYPE_INT8 else (2 if dt == fg.DTYPE_BF16 else 4)
+        Bbuf = torch.empty(w[0] * 4096 * esz, device="cuda", dtype=torch.uint8)  # shared by the set's requests
+        Bbuf.random_(0, 256, generator=gen) if dt == fg.DTYPE_INT8 else \
+            Bbuf.view(torch.bfloat16 if dt == fg.DTYPE_BF16 else torch.float32).normal_(0, 1, generator=gen)
+        sets.append(dict(dtype=int(dt), w=w, As=As, B=Bbuf))
+    set_of = rng.integers(0, len(sets), n_req)
+    Ns = np.minimum(np.exp(rng.uniform(0.0, np.log(4096.0), n_req)).astype(np.int64) + 1, 4096)
+    gaps = rng.exponential(1.0, n_req)  # unit-rate Poisson gaps, scaled per offered rate
+    c_bytes = [sets[i]["w"][-1] * int(n) * 4 for i, n in zip(set_of, Ns)]
+    offs = np.concatenate([[0], np.cumsum([(b + 255) // 256 * 256 for b in c_bytes])])
+    arena = torch.empty(int(offs[-1]), device="cuda", dtype=torch.uint8)  # one C region per request
+    Cs = [arena.data_ptr() + int(o) for o in offs[:-1]]
+    Bs = [sets[i]["B"].data_ptr() for i in set_of]
+    flop = sum(2.0 * int(n) * sum(m * k for m, k in zip(sets[i]["w"][1:], sets[i]["w"][:-1]))
+               for i, n in zip(set_of, Ns))
+    torch.cuda.synchronize()
+    out = dict(requests=n_req, weight_sets=len(sets), total_gflop=flop / 1e9)
+    for qps in rates:
+        arrivals = (np.cumsum(gaps) / qps).tolist()
+        r = {}
+        for mode in ("engine", "per_gemm_baseline"):
+            srv = fg.Server(per_gemm_baseline=(mode != "engine"))
+            try:
+                sids = [srv.add_weights(s["dtype"], s["As"]) for s in sets]
+                t0 = time.perf_counter()
+                srv.run_trace([sids[i] for i in set_of], [int(n) for n in Ns], Bs, Cs, arrivals)
+                wall = time.perf_counter() - t0
+                lat = np.array(srv.latencies()) * 1e3
+            finally:
+                srv.close()
+            r[mode] = dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                           mean_ms=float(lat.mean()), requests_per_s=n_req / wall, tflops=flop / wall / 1e12,
+                           wall_s=wall)
+        r["p99_speedup_vs_per_gemm"] = r["per_gemm_baseline"]["p99_ms"] / r["engine"]["p99_ms"]
+        out[f"qps_{int(qps)}"] = r
+    return out
 
 
 def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
@@ -268,6 +346,7 @@ def main() -> None:
         ("chain3_int8_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_INT8)),
         ("chain3_bf16_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_BF16)),
         ("precision_fp32_4chain", lambda: bench_precision(torch, fg, timer, stream)),
+        ("serving_10k", lambda: bench_serving(torch, fg)),
     )
     if args.only:
         fns = dict(extra)

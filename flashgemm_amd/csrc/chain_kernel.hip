@@ -40,19 +40,19 @@ template <int KIND> struct CT;
 // per wave pass;
 // PF: k-steps per prefetch chunk; FS: packed bytes per lane per fragment.
 template <> struct CT<K_F32> {
-  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, RP = 1, PF = 4, FS = 16;
+  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, RP = 1, PF = 8, FS = 16;
   using Acc = f32x16;
 };
 template <> struct CT<K_BF16> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 2, RP = 2, PF = 4, FS = 16;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 2, RP = 2, PF = 8, FS = 16;
   using Acc = f32x16;
 };
 template <> struct CT<K_I8> {
-  static constexpr int KG = 16, KC = 32, KA = 16, ESZ = 1, RP = 2, PF = 4, FS = 16;
+  static constexpr int KG = 16, KC = 32, KA = 16, ESZ = 1, RP = 2, PF = 8, FS = 16;
   using Acc = i32x16;
 };
 template <> struct CT<K_F32X3> {
-  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, RP = 1, PF = 2, FS = 32;
+  static constexpr int KG = 8, KC = 16, KA = 8, ESZ = 4, RP = 1, PF = 4, FS = 32;
   using Acc = f32x16;
 };
 
@@ -85,10 +85,14 @@ __device__ __forceinline__ void pack_frag(const char *__restrict__ A, long M, lo
                                           int lane) {
   using T = CT<KIND>;
   const int h = lane >> 5, r = lane & 31;
-  const long kspad = kspad_of<KIND>(K);
-  const long i = f % T::RP, ks = (f / T::RP) % kspad, rg = f / (T::RP * kspad);
-  const long row = (rg * T::RP + i) * 32 + r;
-  const long k0 = ks * T::KC + h * T::KA;
+  // 32-bit index math: a packed matrix has far fewer than 2^31 fragments, and a
+  // 64-bit division is a long software sequence on the GPU
+  const unsigned kspad = (unsigned)kspad_of<KIND>(K);
+  const unsigned fu = (unsigned)f, per_rg = kspad * T::RP;
+  const unsigned rg = fu / per_rg, rem = fu - rg * per_rg;
+  const unsigned ks = rem / T::RP, i = rem % T::RP;
+  const long row = (long)(rg * T::RP + i) * 32 + r;
+  const long k0 = (long)ks * T::KC + h * T::KA;
   uint32_t w[T::KA * T::ESZ / 4];
 #pragma unroll
   for (int d = 0; d < T::KA * T::ESZ / 4; ++d) w[d] = 0u;
@@ -407,19 +411,16 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   };
   constexpr size_t CHUNK = (size_t)PF * RP * FRAG;
 
-  // B fragments of one chunk: step s, column block j of this lane at slot
-  // (g0 + 2 s + h) * TN + 32 j + l32 for the chunk's first group g0
-  BFrag<KIND, TNB> fb[2][PF];
+  // B fragments of one k-step: column block j of this lane at slot
+  // (g0 + 2 s + h) * TN + 32 j + l32 for step s of the chunk whose first group is g0
   const uint4 *lane_b = arena + h * TN + l32;
-  auto read_chunk_b = [&](int g0, BFrag<KIND, TNB>(&dst)[PF]) {
-    const uint4 *p = lane_b + g0 * TN;
+  auto read_b = [&](int g0, int s, BFrag<KIND, TNB> &dst) {
+    const uint4 *p = lane_b + (g0 + 2 * s) * TN;
 #pragma unroll
-    for (int s = 0; s < PF; ++s)
-#pragma unroll
-      for (int j = 0; j < TNB; ++j) {
-        dst[s].b[j] = p[2 * s * TN + 32 * j];
-        if (KIND == K_F32X3) dst[s].lo[j] = p[lo_off + 2 * s * TN + 32 * j];
-      }
+    for (int j = 0; j < TNB; ++j) {
+      dst.b[j] = p[32 * j];
+      if (KIND == K_F32X3) dst.lo[j] = p[lo_off + 32 * j];
+    }
   };
 
   // invariant: at the start of each pass fa[0] holds (or has in flight) chunk 0
@@ -438,6 +439,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     // the rows of X
     const int kspad = (int)kspad_of<KIND>(K);
     const int nchunks = kspad / PF;
+    const int ksteps = (int)((K + T::KC - 1) / T::KC);  // k-steps with real data
     if (KIND == K_I8) {
       colsum_lds<TN>(arena, xg0, (int)(kspad * T::KC / T::KG), SG, colsum);
       __syncthreads();
@@ -456,7 +458,6 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
 
     Acc acc[RP][TNB];
     long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
-    if ((long)wave * RP * 32 < Mp) read_chunk_b(chunk_g0(0), fb[0]);
     for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += PASS) {
 #pragma unroll
       for (int i = 0; i < RP; ++i)
@@ -470,31 +471,38 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       // stay unconditional, so their wait counts are exact.
       const char *after = nxt ? nxt : apk;
 
-      auto mma_chunk = [&](const AFrag<KIND>(&a)[PF][RP], const BFrag<KIND, TNB>(&b)[PF]) {
+      // B fragments one step ahead, double buffered (PF is even, so the buffer of
+      // step ks is ks & 1 across chunks); steps past the data (the last chunk's
+      // padding) are skipped with a uniform branch.
+      BFrag<KIND, TNB> bq[2];
+      auto run_chunk = [&](int c, const AFrag<KIND>(&a)[PF][RP]) {
+        const int g0 = chunk_g0(c), gn = chunk_g0(c + 1 < nchunks ? c + 1 : 0);
 #pragma unroll
-        for (int s = 0; s < PF; ++s) mma_k<KIND, TNB>(b[s], a[s], acc);
+        for (int s = 0; s < PF; ++s) {
+          if (c * PF + s >= ksteps) break;
+          if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
+          else read_b(gn, 0, bq[0]);
+          __builtin_amdgcn_sched_barrier(0);
+          mma_k<KIND, TNB>(bq[s & 1], a[s], acc);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       };
-      // The scheduling fences keep each chunk's loads and B reads ahead of the
-      // previous chunk's MFMAs.
+      read_b(chunk_g0(0), 0, bq[0]);
+      // The scheduling fences keep each chunk's weight loads ahead of the previous
+      // chunk's MFMAs.
       int c = 0;
       for (; c + 1 < nchunks; c += 2) {
         load_chunk(apk + (c + 1) * CHUNK, fa[1]);
-        read_chunk_b(chunk_g0(c + 1), fb[1]);
         __builtin_amdgcn_sched_barrier(0);
-        mma_chunk(fa[0], fb[0]);
+        run_chunk(c, fa[0]);
+        load_chunk(c + 2 < nchunks ? apk + (c + 2) * CHUNK : after, fa[0]);
         __builtin_amdgcn_sched_barrier(0);
-        const bool more = c + 2 < nchunks;
-        load_chunk(more ? apk + (c + 2) * CHUNK : after, fa[0]);
-        read_chunk_b(chunk_g0(more ? c + 2 : 0), fb[0]);
-        __builtin_amdgcn_sched_barrier(0);
-        mma_chunk(fa[1], fb[1]);
-        __builtin_amdgcn_sched_barrier(0);
+        run_chunk(c + 1, fa[1]);
       }
       if (c < nchunks) {  // odd chunk count: the next loads wait for this chunk
-        mma_chunk(fa[0], fb[0]);
+        run_chunk(c, fa[0]);
         __builtin_amdgcn_sched_barrier(0);
         load_chunk(after, fa[0]);
-        read_chunk_b(chunk_g0(0), fb[0]);
       }
       apk = after;
       if (final_pass && !last) held_rb0 = rb0;
@@ -674,8 +682,9 @@ size_t packed_a_bytes(int kind, long M, long K) {
 
 hipError_t launch_pack_a(int kind, const PackJob &J, hipStream_t s) {
   if (J.n < 1 || J.n > PACK_MAX) return hipErrorInvalidValue;
-  for (int m = 0; m < J.n; ++m)
-    if (J.M[m] <= 0 || J.K[m] <= 0) return hipErrorInvalidValue;
+  for (int m = 0; m < J.n; ++m)  // positive shapes; fragment indices fit 32 bits
+    if (J.M[m] <= 0 || J.K[m] <= 0 || packed_a_bytes(kind, J.M[m], J.K[m]) / 1024 >= (1ull << 31))
+      return hipErrorInvalidValue;
   switch (kind) {
     case K_F32: return launch_pack_t<K_F32>(J, s);
     case K_BF16: return launch_pack_t<K_BF16>(J, s);

@@ -382,41 +382,51 @@ __global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, c
 }
 
 // INT8 column sums sum_k s8(B[k][n]) for the u8 bias term (exact in int32 for
-// K < 2^24). A thread sums CS_ROWS rows of 16 adjacent columns (one 16-byte load per
-// row, coalesced across the wave); the row blocks meet in integer atomics, which are
-// exact and order independent, so the result is deterministic. out[] must be zero
-// first (colsum_zero_kernel, a fill kernel rather than a memset so the pair can sit
-// inside a captured graph).
-constexpr int CS_ROWS = 16;
+// K < 2^24). A workgroup covers 256 columns x CS_ROWS rows: thread (ty, tx) sums
+// CS_RPT rows of 16 adjacent columns (one 16-byte load per row, 16 threads cover
+// 256 contiguous bytes of a row), the 16 row slices are added in LDS and each
+// column meets the other row blocks in one integer atomic (exact and order
+// independent, so the result is deterministic). out[] must be zero first
+// (colsum_zero_kernel, a fill kernel rather than a memset so the pair can sit inside
+// a captured graph).
+constexpr int CS_RPT = 8;
+constexpr int CS_ROWS = 16 * CS_RPT;
 
 __global__ void colsum_zero_kernel(int *__restrict__ out, long N) {
   long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n < N) out[n] = 0;
 }
 
-__global__ void colsum_i8_kernel(const int8_t *__restrict__ B, long K, long N, int *__restrict__ out) {
-  const long n = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 16;
-  const long k0 = (long)blockIdx.y * CS_ROWS;
-  if (n >= N) return;
-  const long k1 = (k0 + CS_ROWS < K) ? k0 + CS_ROWS : K;
+__global__ __launch_bounds__(256) void colsum_i8_kernel(const int8_t *__restrict__ B, long K, long N,
+                                                        int *__restrict__ out) {
+  __shared__ int part[16][256];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const long n = (long)blockIdx.x * 256 + tx * 16;
+  const long k0 = (long)blockIdx.y * CS_ROWS + ty * CS_RPT;
   int s[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) s[c] = 0;
-  if (n + 16 <= N && (N & 15) == 0 && ((uintptr_t)B & 15) == 0 && k1 - k0 == CS_ROWS) {
+  if (n + 16 <= N && (N & 15) == 0 && ((uintptr_t)B & 15) == 0 && k0 + CS_RPT <= K) {
 #pragma unroll
-    for (int r = 0; r < CS_ROWS; ++r) {
+    for (int r = 0; r < CS_RPT; ++r) {
       const uint4 w = *(const uint4 *)(B + (size_t)(k0 + r) * N + n);
       const uint32_t q[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
       for (int c = 0; c < 16; ++c) s[c] += (int)(int8_t)(q[c >> 2] >> (8 * (c & 3)));
     }
   } else {
-    for (long k = k0; k < k1; ++k)
+    for (long k = k0; k < k0 + CS_RPT && k < K; ++k)
       for (int c = 0; c < 16; ++c)
         if (n + c < N) s[c] += B[(size_t)k * N + n + c];
   }
-  for (int c = 0; c < 16; ++c)
-    if (n + c < N) atomicAdd(&out[n + c], s[c]);
+#pragma unroll
+  for (int c = 0; c < 16; ++c) part[ty][tx * 16 + c] = s[c];
+  __syncthreads();
+  const long col = (long)blockIdx.x * 256 + threadIdx.x;
+  if (col >= N) return;
+  int t = 0;
+  for (int y = 0; y < 16; ++y) t += part[y][threadIdx.x];
+  atomicAdd(&out[col], t);
 }
 
 // Split-K reduction: a workgroup owns RE consecutive elements; RL lanes per element
@@ -508,7 +518,7 @@ hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
 
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s) {
   hipLaunchKernelGGL(colsum_zero_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, out, N);
-  dim3 grid((unsigned)((N + 16 * 256 - 1) / (16 * 256)), (unsigned)((K + CS_ROWS - 1) / CS_ROWS));
+  dim3 grid((unsigned)((N + 255) / 256), (unsigned)((K + CS_ROWS - 1) / CS_ROWS));
   hipLaunchKernelGGL(colsum_i8_kernel, grid, dim3(256), 0, s, B, K, N, out);
   return hipGetLastError();
 }

@@ -3,8 +3,10 @@
 // Replaces the reference's packing micro-kernels and register-blocked AVX-512
 // kernels (singlegemm/src/kernel.h, kernelMGN.h, PACK_x86.h, PACK_B.h;
 // multigemm/src/kernel_*.h) with one workgroup-level design:
-//   * 256 threads = 4 waves in a 2 x 2 grid, 128 x 128 output tile, each wave
-//     64 x 64 = 2 x 2 MFMA 32x32 blocks;
+//   * two tile shapes: 256 threads = 4 waves in a 2 x 2 grid on a 128 x 128 output
+//     tile (each wave 64 x 64 = 2 x 2 MFMA 32x32 blocks), and for large problems
+//     512 threads = 8 waves in a 2 x 4 grid on a 256 x 256 tile (each wave
+//     128 x 64 = 4 x 2 blocks), which halves the operand bytes per MFMA;
 //   * A and B staged global -> registers -> LDS, double buffered, in the MFMA
 //     operand layout: 16-byte k-groups per row (A) or per column (B), slot index
 //     g*rows + (r ^ (g & 7)) so both the staging writes and the fragment reads
@@ -22,9 +24,16 @@
 namespace fg {
 namespace {
 
-constexpr int BM = 128;
-constexpr int BN = 128;
-constexpr int NT = 256;
+// Tile configurations: BT x BT output tile, WM x WN waves, each wave a
+// (BT/WM) x (BT/WN) block of 32 x 32 MFMA tiles (IM x JN of them).
+template <int BT_, int WM_, int WN_> struct Tile {
+  static constexpr int BT = BT_, WM = WM_, WN = WN_, NW = WM_ * WN_, NT = NW * 64;
+  static constexpr int IM = BT_ / WM_ / 32, JN = BT_ / WN_ / 32;
+  // one operand tile per stage: 8 k-groups of 16 bytes x BT rows (both planes for F32X3)
+  static constexpr int TU4 = 8 * BT_;
+};
+using TileS = Tile<128, 2, 2>;  // 256 threads, 64 x 64 per wave, 64 KB LDS
+using TileL = Tile<256, 2, 4>;  // 512 threads, 128 x 64 per wave, 128 KB LDS
 
 template <int KIND> struct KT;
 // BK: k per pipeline stage. KG: k per 16-byte LDS group (per plane for F32X3).
@@ -47,23 +56,23 @@ template <> struct KT<K_F32X3> {
   using Acc = f32x16;
 };
 
-// One operand tile per stage is 16 KB: BK/KG groups x 128 rows x 16 B, or for
-// F32X3 two planes (hi, lo) of BK/8 groups x 128 rows x 16 B.
-constexpr int TILE_U4 = 16384 / 16;
-
-__device__ __forceinline__ int slot(int g, int r) { return g * 128 + (r ^ (g & 7)); }
+// Operand tile per stage: BK/KG groups x BT rows x 16 B, or for F32X3 two planes
+// (hi, lo) of BK/8 groups x BT rows x 16 B: 8 x BT 16-byte slots either way.
+template <int BT>
+__device__ __forceinline__ int slot(int g, int r) { return g * BT + (r ^ (g & 7)); }
 
 // ---------------------------------------------------------------- staging --
-template <int KIND>
+template <int KIND, class TL>
 struct Stager {
   using T = KT<KIND>;
+  static constexpr int NT = TL::NT, BT = TL::BT, TILE_U4 = TL::TU4;
   uint32_t a[16];  // 4 x 16 B of A
   uint32_t b[16];  // RB rows x CW columns of B
 
-  // A tile rows [m0, m0+128) x k [k0, k0+BK): 1024 16-byte units, 4 per thread.
+  // A tile rows [m0, m0+BT) x k [k0, k0+BK): 8 BT 16-byte units, 4 per thread.
   __device__ __forceinline__ void load_a(const char *A, long M, long K, long kend, long m0, long k0,
                                          bool vec, int t) {
-    if (vec && m0 + 128 <= M && k0 + T::BK <= kend) {  // interior tile (uniform branch)
+    if (vec && m0 + BT <= M && k0 + T::BK <= kend) {  // interior tile (uniform branch)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         int u = t + NT * q;
@@ -122,25 +131,25 @@ struct Stager {
         split_pk(__builtin_bit_cast(float, a[4 * q]), __builtin_bit_cast(float, a[4 * q + 1]), h01, l01);
         split_pk(__builtin_bit_cast(float, a[4 * q + 2]), __builtin_bit_cast(float, a[4 * q + 3]), h23, l23);
         int g = v >> 1, half = v & 1;
-        uint2 *hp = (uint2 *)(sA + slot(g, row)) + half;
-        uint2 *lp = (uint2 *)(sA + TILE_U4 / 2 + slot(g, row)) + half;
+        uint2 *hp = (uint2 *)(sA + slot<BT>(g, row)) + half;
+        uint2 *lp = (uint2 *)(sA + TILE_U4 / 2 + slot<BT>(g, row)) + half;
         *hp = make_uint2(h01, h23);
         *lp = make_uint2(l01, l23);
       } else {
-        sA[slot(v, row)] = make_uint4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
+        sA[slot<BT>(v, row)] = make_uint4(a[4 * q], a[4 * q + 1], a[4 * q + 2], a[4 * q + 3]);
       }
     }
   }
 
-  // B tile k [k0, k0+BK) x columns [n0, n0+128): thread = RB rows x CW columns.
+  // B tile k [k0, k0+BK) x columns [n0, n0+BT): thread = RB rows x CW columns.
   __device__ __forceinline__ void load_b(const char *B, long N, long kend, long n0, long k0, bool vec,
                                          int t) {
-    constexpr int NCB = 128 / T::CW;  // column blocks
+    constexpr int NCB = BT / T::CW;  // column blocks
     int cb = t % NCB, kg = t / NCB;
     long n = n0 + (long)cb * T::CW;
     long kb = k0 + (long)kg * T::RB;
     constexpr int RBYTES = T::CW * T::ESZ;  // bytes per row piece: 16, 8, 4 or 8
-    if (vec && n0 + 128 <= N && k0 + T::BK <= kend) {  // interior tile (uniform branch)
+    if (vec && n0 + BT <= N && k0 + T::BK <= kend) {  // interior tile (uniform branch)
 #pragma unroll
       for (int r = 0; r < T::RB; ++r) {
         const char *p = B + ((size_t)(kb + r) * N + n) * T::ESZ;
@@ -193,13 +202,13 @@ struct Stager {
   }
 
   __device__ __forceinline__ void store_b(uint4 *sB, int t) const {
-    constexpr int NCB = 128 / T::CW;
+    constexpr int NCB = BT / T::CW;
     int cb = t % NCB, g = t / NCB;
 #pragma unroll
     for (int c = 0; c < T::CW; ++c) {
       int n = cb * T::CW + c;
       if (KIND == K_F32) {
-        sB[slot(g, n)] = make_uint4(b[c], b[4 + c], b[8 + c], b[12 + c]);
+        sB[slot<BT>(g, n)] = make_uint4(b[c], b[4 + c], b[8 + c], b[12 + c]);
       } else if (KIND == K_BF16) {
         uint32_t w[4];
         for (int d = 0; d < 4; ++d) {
@@ -207,7 +216,7 @@ struct Stager {
           uint32_t hi = (b[2 * (2 * d + 1) + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
           w[d] = lo | (hi << 16);
         }
-        sB[slot(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
+        sB[slot<BT>(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
       } else if (KIND == K_I8) {
         uint32_t w[4];
         for (int d = 0; d < 4; ++d) {
@@ -215,38 +224,43 @@ struct Stager {
           for (int e = 0; e < 4; ++e) x |= ((b[4 * d + e] >> (8 * c)) & 0xffu) << (8 * e);
           w[d] = x;
         }
-        sB[slot(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
+        sB[slot<BT>(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
       } else {  // F32X3: 8 floats of column c -> hi and lo planes
         uint32_t h[4], l[4];
         for (int r = 0; r < 4; ++r)
           split_pk(__builtin_bit_cast(float, b[4 * r + c]), __builtin_bit_cast(float, b[4 * r + 2 + c]), h[r], l[r]);
-        sB[slot(g, n)] = make_uint4(h[0], h[1], h[2], h[3]);
-        sB[TILE_U4 / 2 + slot(g, n)] = make_uint4(l[0], l[1], l[2], l[3]);
+        sB[slot<BT>(g, n)] = make_uint4(h[0], h[1], h[2], h[3]);
+        sB[TILE_U4 / 2 + slot<BT>(g, n)] = make_uint4(l[0], l[1], l[2], l[3]);
       }
     }
   }
 };
 
 // ------------------------------------------------------------------- MFMA --
-template <int KIND>
-__device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int kc, int wm, int wn,
-                                         int lane, typename KT<KIND>::Acc (&acc)[2][2]) {
+template <int KIND, class TL>
+__device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int kc, int wm, int wn, int lane,
+                                         typename KT<KIND>::Acc (&acc)[TL::IM][TL::JN]) {
   using T = KT<KIND>;
+  constexpr int BT = TL::BT, IM = TL::IM, JN = TL::JN, HP = TL::TU4 / 2;
   const int h = lane >> 5, l32 = lane & 31;
+  const int r0 = wm * (BT / TL::WM) + l32, c0 = wn * (BT / TL::WN) + l32;
   if (KIND == K_F32X3) {
     const int g = kc / 8 + h;
-    uint4 ah[2], al[2], bh[2], bl[2];
+    uint4 ah[IM], al[IM], bh[JN], bl[JN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      ah[i] = sA[slot(g, wm * 64 + i * 32 + l32)];
-      al[i] = sA[TILE_U4 / 2 + slot(g, wm * 64 + i * 32 + l32)];
-      bh[i] = sB[slot(g, wn * 64 + i * 32 + l32)];
-      bl[i] = sB[TILE_U4 / 2 + slot(g, wn * 64 + i * 32 + l32)];
+    for (int i = 0; i < IM; ++i) {
+      ah[i] = sA[slot<BT>(g, r0 + i * 32)];
+      al[i] = sA[HP + slot<BT>(g, r0 + i * 32)];
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < JN; ++j) {
+      bh[j] = sB[slot<BT>(g, c0 + j * 32)];
+      bl[j] = sB[HP + slot<BT>(g, c0 + j * 32)];
+    }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j) {
         // small terms first, then the dominant hi.hi product
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al[i]),
                                                             __builtin_bit_cast(bf16x8, bh[j]), acc[i][j], 0, 0, 0);
@@ -258,16 +272,15 @@ __device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int k
     return;
   }
   const int g = kc / T::KG + h;
-  uint4 a[2], b[2];
+  uint4 a[IM], b[JN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    a[i] = sA[slot(g, wm * 64 + i * 32 + l32)];
-    b[i] = sB[slot(g, wn * 64 + i * 32 + l32)];
-  }
+  for (int i = 0; i < IM; ++i) a[i] = sA[slot<BT>(g, r0 + i * 32)];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < JN; ++j) b[j] = sB[slot<BT>(g, c0 + j * 32)];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
       if (KIND == K_F32) {
         // lane half h holds k = kc + 4h + s for s = 0..3 in both operands
         f32x4 av = __builtin_bit_cast(f32x4, a[i]);
@@ -296,33 +309,34 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &
   tn = L / tiles_m;
 }
 
-template <int KIND, int EPI>
-__global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
-                                                  void *__restrict__ Cv, const int *__restrict__ colsum,
-                                                  long M, long N, long K, long kper, int tiles_m, int tiles_n,
-                                                  int vecA, int vecB) {
+template <int KIND, int EPI, class TL>
+__global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
+                                                      void *__restrict__ Cv, const int *__restrict__ colsum,
+                                                      long M, long N, long K, long kper, int tiles_m, int tiles_n,
+                                                      int vecA, int vecB) {
   using T = KT<KIND>;
   using Acc = typename T::Acc;
-  __shared__ uint4 smem[4 * TILE_U4];  // [stage][A|B]
+  constexpr int BT = TL::BT, IM = TL::IM, JN = TL::JN, TILE_U4 = TL::TU4;
+  extern __shared__ uint4 smem[];  // [stage][A|B], 4 x TILE_U4 slots
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / TL::WN, wn = wave % TL::WN;
   int tm, tn;
   tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long m0 = (long)tm * BT, n0 = (long)tn * BT;
   const long kbeg = (long)blockIdx.y * kper;
   const long kend = (kbeg + kper < K) ? kbeg + kper : K;
 
-  Acc acc[2][2];
+  Acc acc[IM][JN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < JN; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0;
 
   const char *A = (const char *)Av;
   const char *B = (const char *)Bv;
-  Stager<KIND> st;
+  Stager<KIND, TL> st;
   int nstages = (int)((kend - kbeg + T::BK - 1) / T::BK);
   if (nstages > 0) {
     st.load_a(A, M, K, kend, m0, kbeg, vecA, t);
@@ -341,7 +355,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, c
       st.load_b(B, N, kend, n0, k0, vecB, t);
     }
 #pragma unroll
-    for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
+    for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND, TL>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
     if (more) {
       st.store_a(nxt, t);
       st.store_b(nxt + TILE_U4, t);
@@ -352,16 +366,16 @@ __global__ __launch_bounds__(NT) void gemm_kernel(const void *__restrict__ Av, c
   // ---- epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column lane%32 per block
   const int h = lane >> 5, l32 = lane & 31;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    long n = n0 + wn * 64 + j * 32 + l32;
+  for (int j = 0; j < JN; ++j) {
+    long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
     if (n >= N) continue;
     int bias = 0;
     if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * colsum[n];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < IM; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        long m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        long m = m0 + wm * (BT / TL::WM) + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
         if (m >= M) continue;
         size_t off = (size_t)m * N + n;
         if (EPI == EPI_PARTIAL) {
@@ -477,20 +491,33 @@ __global__ void ref_permute_kernel(const E *__restrict__ Z, E *__restrict__ X, l
   X[e] = Z[(size_t)r * p.N + p0 + c];
 }
 
-template <int KIND, int EPI>
-hipError_t launch_one(const GemmArgs &g, hipStream_t s) {
+template <int KIND, int EPI, class TL>
+hipError_t launch_tile(const GemmArgs &g, hipStream_t s) {
   using T = KT<KIND>;
-  int tiles_m = (int)((g.M + BM - 1) / BM), tiles_n = (int)((g.N + BN - 1) / BN);
+  int tiles_m = (int)((g.M + TL::BT - 1) / TL::BT), tiles_n = (int)((g.N + TL::BT - 1) / TL::BT);
   int splits = g.splits > 0 ? g.splits : 1;
   long kper = ((g.K + splits - 1) / splits + T::BK - 1) / T::BK * T::BK;
   splits = (int)((g.K + kper - 1) / kper);
   int esz = T::ESZ;
   bool vecA = ((g.K * esz) % 16 == 0) && (((uintptr_t)g.A & 15) == 0);
   bool vecB = (g.N % T::CW == 0) && (((uintptr_t)g.B & (T::CW * esz - 1)) == 0);
+  const size_t lds = 4 * (size_t)TL::TU4 * sizeof(uint4);
+  auto kfn = gemm_kernel<KIND, EPI, TL>;
+  hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
   dim3 grid(tiles_m * tiles_n, splits);
-  hipLaunchKernelGGL((gemm_kernel<KIND, EPI>), grid, dim3(NT), 0, s, g.A, g.B, g.C, g.colsum, g.M, g.N, g.K,
-                     kper, tiles_m, tiles_n, (int)vecA, (int)vecB);
+  hipLaunchKernelGGL(kfn, grid, dim3(TL::NT), lds, s, g.A, g.B, g.C, g.colsum, g.M, g.N, g.K, kper, tiles_m, tiles_n,
+                     (int)vecA, (int)vecB);
   return hipGetLastError();
+}
+
+// The 256 x 256 tile reads half the operand bytes per MFMA of the 128 x 128 tile;
+// it is used when it still gives about one tile per CU.
+template <int KIND, int EPI>
+hipError_t launch_one(const GemmArgs &g, hipStream_t s) {
+  const long big = ((g.M + 255) / 256) * ((g.N + 255) / 256) * (g.splits > 1 ? g.splits : 1);
+  if (big >= 192) return launch_tile<KIND, EPI, TileL>(g, s);
+  return launch_tile<KIND, EPI, TileS>(g, s);
 }
 
 template <int KIND>

@@ -85,6 +85,35 @@ def test_sgemm_fp32_mfma_vs_oracle(gpu, M, N, K):
     assert O.rel_fro(got, A.astype(np.float64) @ B.astype(np.float64)) <= TOL_F32
 
 
+@pytest.mark.parametrize("f32_mode", [fg.F32_MFMA, fg.F32_BF16X3])
+def test_sgemm_large_tile_vs_f64(gpu, f32_mode):
+    """Enough 256 x 256 tiles for the large-tile kernel, ragged in M and N."""
+    import torch
+    M, N, K = 4000, 3500, 200
+    print(f"sgemm large tile {M}x{N}x{K} mode={f32_mode}", flush=True)
+    rng = np.random.default_rng(41)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    C = empty((M, N), torch.float32)
+    fg.sgemm(C, dev(A), dev(B), M, N, K, f32_mode=f32_mode)
+    torch.cuda.synchronize()
+    tol = TOL_F32 if f32_mode == fg.F32_MFMA else TOL_X3
+    assert O.rel_fro(C.cpu().numpy(), A.astype(np.float64) @ B.astype(np.float64)) <= tol
+
+
+@pytest.mark.parametrize("dtype", [fg.DTYPE_INT8, fg.DTYPE_BF16])
+def test_unfused_chain_large_tile(gpu, dtype):
+    """A 2000 x 6000 first GEMM runs on the large-tile kernel (unfused chain)."""
+    print(f"unfused large-tile chain dtype={dtype}", flush=True)
+    rng = np.random.default_rng(43)
+    B, As = rand_chain(rng, dtype, 300, 6000, [2000, 96])
+    got = run_chain(dtype, B, As, fuse=fg.FUSE_OFF)
+    if dtype == fg.DTYPE_INT8:
+        assert np.array_equal(got, O.chain_int8(B, As, O.LAYOUT_MATH))
+    else:
+        assert O.rel_fro(got, O.chain_bf16(B, As, O.LAYOUT_MATH)) <= TOL_BF16
+
+
 @pytest.mark.parametrize("M,N,K,splits", [(64, 64, 4096, 8), (200, 40, 3000, 5), (64, 64, 65536, 0)])
 def test_sgemm_split_k_deterministic(gpu, M, N, K, splits):
     import torch

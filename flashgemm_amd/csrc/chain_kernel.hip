@@ -59,6 +59,16 @@ template <> struct CT<K_F32X3> {
 template <int TN>
 __device__ __forceinline__ int cslot(int g, int n) { return g * TN + n; }
 
+// threadIdx.x passed through an empty asm: addresses derived from it are recomputed
+// where they are used (two VALU ops) instead of being hoisted out of the GEMM loop and
+// kept live, or spilled to scratch, across the whole chain. A spill reload would also
+// wait for every weight load in flight.
+__device__ __forceinline__ int fresh_tid() {
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+}
+
 // --------------------------------------------------------- packed weights --
 // Packed layout of A (M x K) for kind KIND: fragments f = (rg * kspad + ks) * RP + i
 // for row group rg (RP * 32 rows), k-step ks < kspad and row block i; fragment f is
@@ -256,10 +266,11 @@ template <int TN>
 __device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int SG, int *colsum) {
   // thread -> (column, slice of groups); reduce slices with LDS atomics (integer adds
   // are order independent, so the result is deterministic).
-  for (int n = threadIdx.x; n < TN; n += NTH) colsum[n] = 0;
+  const int tid = fresh_tid();
+  for (int n = tid; n < TN; n += NTH) colsum[n] = 0;
   __syncthreads();
   constexpr int SL = NTH / TN;  // slices per column
-  int n = threadIdx.x % TN, sl = threadIdx.x / TN;
+  int n = tid % TN, sl = tid / TN;
   if (sl < SL) {
     int s = 0;
     for (int g = sl; g < ngroups; g += SL) {
@@ -287,7 +298,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   const int S = P.S;
   const int SG = S / T::KG;
   const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;
-  const int lane = threadIdx.x & 63;
+  const int lane = fresh_tid() & 63;
   const int h = lane >> 5, l32 = lane & 31;
   const int M = (int)P.M[gi], Mp = (M + 31) & ~31;
   const int rb0 = (int)rb0_l;
@@ -393,11 +404,14 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   const int h = lane >> 5, l32 = lane & 31;
 
   // first packed-weight chunk of this wave in GEMM gi (nullptr: no rows there);
-  // fragment (ks, i) of a pass at base + (ks * RP + i) * FRAG
+  // fragment (ks, i) of a pass at base + (ks * RP + i) * FRAG. The chunk pointers are
+  // wave-uniform (scalar registers); each load adds the lane's 16-byte offset, so
+  // the weight stream costs one vector register for addressing.
   auto first_ptr = [&](int gi) -> const char * {
     if ((long)wave * RP * 32 >= (P.M[gi] + 31) / 32 * 32) return nullptr;
-    return (const char *)P.A[gi] + (size_t)wave * kspad_of<KIND>(P.K[gi]) * RP * FRAG + lane * 16;
+    return (const char *)P.A[gi] + (size_t)wave * kspad_of<KIND>(P.K[gi]) * RP * FRAG;
   };
+  const unsigned lane16 = (unsigned)lane * 16u;
   AFrag<KIND> fa[2][PF][RP];
   auto load_chunk = [&](const char *base, AFrag<KIND>(&dst)[PF][RP]) {
 #pragma unroll
@@ -405,8 +419,8 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
 #pragma unroll
       for (int i = 0; i < RP; ++i) {
         const char *p = base + (size_t)(s * RP + i) * FRAG;
-        dst[s][i].v[0] = *(const uint4 *)p;
-        if (KIND == K_F32X3) dst[s][i].v[1] = *(const uint4 *)(p + 1024);
+        dst[s][i].v[0] = *(const uint4 *)(p + lane16);
+        if (KIND == K_F32X3) dst[s][i].v[1] = *(const uint4 *)(p + 1024 + lane16);
       }
   };
   constexpr size_t CHUNK = (size_t)PF * RP * FRAG;
@@ -454,7 +468,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     // so the MFMAs accumulate u8(a).b directly.
     int bias0[TNB];
 #pragma unroll
-    for (int j = 0; j < TNB; ++j) bias0[j] = (KIND == K_I8) ? 128 * colsum[j * 32 + l32] : 0;
+    for (int j = 0; j < TNB; ++j) bias0[j] = (KIND == K_I8) ? 128 * colsum[j * 32 + (fresh_tid() & 31)] : 0;
 
     Acc acc[RP][TNB];
     long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
@@ -519,7 +533,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     {
       const long Mc = kspad_of<KIND>(M) * T::KC;
       const int zg0 = (int)((P.ybase[gi] + Mp) / T::KG), nzg = (int)((Mc - Mp) / T::KG);
-      for (int t = threadIdx.x; t < nzg * TN; t += NTH) {
+      for (int t = fresh_tid(); t < nzg * TN; t += NTH) {
         const int g = zg0 + t / TN < SG ? zg0 + t / TN : zg0 + t / TN - SG;
         arena[cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
         if (KIND == K_F32X3) arena[lo_off + cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);

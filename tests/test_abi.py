@@ -73,3 +73,20 @@ def test_drivers_link(tmp_path, built_lib):
         subprocess.run(["gcc", "-O2", "-std=gnu11", "-Wall", "-I", os.path.join(ROOT, "include"),
                         os.path.join(drivers, f), "-o", str(exe), "-L", os.path.dirname(LIB), "-lflashgemm",
                         f"-Wl,-rpath,{os.path.dirname(LIB)}", "-lm"], check=True)
+
+
+def test_packed_weight_bytes(built_lib):
+    """Packed weights are whole 1 KiB MFMA fragments (2 KiB for the split-bf16 form); a
+    shape that fills every fragment packs to exactly its own bytes, a ragged one is
+    zero padded up, and invalid arguments size to 0 (no device needed)."""
+    import flashgemm_amd as fg
+    for dtype, mode, esz in ((fg.DTYPE_INT8, fg.F32_MFMA, 1), (fg.DTYPE_BF16, fg.F32_MFMA, 2),
+                             (fg.DTYPE_F32, fg.F32_MFMA, 4), (fg.DTYPE_F32, fg.F32_BF16X3, 4)):
+        assert fg.packed_weight_bytes(dtype, 512, 512, mode) == 512 * 512 * esz
+        assert fg.packed_weight_bytes(dtype, 2048, 1024, mode) == 2048 * 1024 * esz
+        for M, K in ((1, 1), (33, 17), (130, 1000)):
+            got = fg.packed_weight_bytes(dtype, M, K, mode)
+            assert got >= M * K * esz and got % 1024 == 0
+        assert fg.packed_weight_bytes(dtype, 0, 64, mode) == 0
+        assert fg.packed_weight_bytes(dtype, 64, -1, mode) == 0
+    assert fg.packed_weight_bytes(7, 64, 64) == 0

@@ -69,6 +69,34 @@ def test_grouped_mixed_batch_matches_oracle_and_single_launch(gpu):
         assert np.array_equal(got.view(np.uint32), single_launch(q).view(np.uint32))
 
 
+def test_grouped_prepacked_requests_same_bits(gpu):
+    """Requests whose weights were packed beforehand (a_packed=1) ride in the same
+    grouped launch as ones the launch packs itself, with the same bits as before."""
+    import torch
+    rng = np.random.default_rng(17)
+    reqs = make_requests(rng, 12, fitting=True)
+    fg.grouped_chain([q["req"] for q in reqs])
+    torch.cuda.synchronize()
+    want = [q["C"].cpu().numpy() for q in reqs]
+    keep, mixed = [], []
+    for j, q in enumerate(reqs):
+        q["C"].zero_()
+        if j % 2:
+            mixed.append(q["req"])
+            continue
+        Ms, Ks = [a.shape[0] for a in q["As"]], [a.shape[1] for a in q["As"]]
+        packed = [empty((fg.packed_weight_bytes(q["dtype"], m, k),), torch.uint8) for m, k in zip(Ms, Ks)]
+        for a, m, k, p in zip(q["dA"], Ms, Ks, packed):
+            fg.pack_weight(q["dtype"], a, m, k, p)
+        keep.append(packed)
+        mixed.append(fg.chain_req(q["dtype"], q["C"], q["dB"], packed, N=q["N"], Ms=Ms, Ks=Ks, a_packed=1))
+    print("grouped batch, half pre-packed:", len(mixed), flush=True)
+    fg.grouped_chain(mixed)
+    torch.cuda.synchronize()
+    for q, w in zip(reqs, want):
+        assert np.array_equal(q["C"].cpu().numpy().view(np.uint32), w.view(np.uint32))
+
+
 def test_grouped_oversized_request_runs_unfused(gpu):
     import torch
     rng = np.random.default_rng(3)

@@ -283,6 +283,37 @@ def test_f32_chain_split_bf16x3(gpu, fuse):
     assert O.rel_fro(got, ref) <= TOL_X3
 
 
+@pytest.mark.parametrize("dtype,f32_mode", [(fg.DTYPE_INT8, fg.F32_MFMA), (fg.DTYPE_BF16, fg.F32_MFMA),
+                                            (fg.DTYPE_F32, fg.F32_MFMA), (fg.DTYPE_F32, fg.F32_BF16X3)])
+def test_prepacked_weights_same_bits(gpu, dtype, f32_mode):
+    """Weights packed once (fg_pack_weight) and passed with a_packed=1 give the same bits
+    as the call that packs them itself; a shape the fused path cannot take is refused."""
+    import torch
+    print(f"prepacked dtype={dtype} f32_mode={f32_mode}", flush=True)
+    rng = np.random.default_rng(10 * dtype + f32_mode)
+    K1, N, Ms = 96, 300, [130, 48, 72]
+    Ks = [K1] + Ms[:-1]
+    B, As = rand_chain(rng, dtype, K1, N, Ms, full_int8=True)
+    want = run_chain(dtype, B, As, fuse=fg.FUSE_ON, f32_mode=f32_mode)
+    dA = [dev(a) for a in As]
+    packed = [empty((fg.packed_weight_bytes(dtype, m, k, f32_mode),), torch.uint8) for m, k in zip(Ms, Ks)]
+    for a, m, k, p in zip(dA, Ms, Ks, packed):
+        fg.pack_weight(dtype, a, m, k, p, f32_mode=f32_mode)
+    C = empty((Ms[-1], N), torch.int32 if dtype == fg.DTYPE_INT8 else torch.float32)
+    dB = dev(B)
+    fg.chain(dtype, C, dB, packed, N=N, Ms=Ms, Ks=Ks, o=fg.opts(f32_mode=f32_mode, a_packed=1))
+    torch.cuda.synchronize()
+    got = C.cpu().numpy()
+    assert np.array_equal(got, want), f"{(got != want).sum()} of {got.size} differ"
+    if dtype == fg.DTYPE_INT8:
+        assert np.array_equal(got, O.chain_int8(B, As, O.LAYOUT_MATH))
+    # N % 4 != 0 has no fused form, and packed weights have no other consumer
+    with pytest.raises(fg.FlashgemmError) as e:
+        fg.chain(dtype, C, dB, packed, N=N - 1, Ms=Ms, Ks=Ks, o=fg.opts(f32_mode=f32_mode, a_packed=1))
+    assert e.value.status == fg.ERR_UNSUPPORTED
+    torch.cuda.synchronize()
+
+
 @pytest.mark.parametrize("threads,N,Ms", [(1, 96, [24, 36, 12]), (4, 160, [12, 24, 36]), (2, 64, [36, 48, 24, 12]),
                                           (3, 96, [24, 12])])
 def test_int8_reference_layout_device_api(gpu, threads, N, Ms):

@@ -21,8 +21,9 @@
 //   * each wave owns row blocks of the weights and streams their packed fragments
 //     as one sequence, PF k-steps per chunk, the next chunk in flight across row
 //     passes and GEMM boundaries; B fragments are read one step ahead;
-//   * INT8 uses u8(a).b = s8(a ^ 0x80).b + 128 sum_k b: the column sums of each X_i
-//     are taken from LDS and seed the accumulators;
+//   * INT8 uses u8(a).b = s8(a ^ 0x80).b + 128 sum_k b: the column sums of X_0 are
+//     taken while the tile is staged and those of each Y_i by the epilogue that
+//     writes it, and they seed the next GEMM's accumulators;
 //   * only the last GEMM writes HBM (C, fp32 or int32, row-major).
 #include "fg_internal.h"
 
@@ -201,42 +202,72 @@ __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KI
 // Stage the B tile (rows [0, K1p) x columns [n0, n0+TN)) into the arena at row 0.
 // The fused path requires N % 4 == 0 and a 4-element aligned B, so each 4-column
 // group lies entirely inside or outside the matrix; out-of-range rows and groups
-// read a clamped address and are zeroed with a select.
-template <int KIND, int TN>
-__device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K1p, long N, long n0) {
+// read a clamped address and are zeroed with a select. `issue` runs after the first
+// round of tile loads and before their LDS writes: the caller's weight prefetch goes
+// there, so the tile's loads are not queued behind it. INT8 adds each column's sum of
+// s8 X_0 into colsum (zeroed beforehand).
+template <int KIND, int TN, typename F>
+__device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K1p, long N, long n0,
+                                        int *colsum, F &&issue) {
   using T = CT<KIND>;
   // one task = KG rows x 4 columns of one k-group
   const int ngroups = (int)(K1p / T::KG);
   const int tasks = ngroups * (TN / 4);
-  for (int task = threadIdx.x; task < tasks; task += NTH) {
-    int g = task / (TN / 4), cb = task % (TN / 4);
-    const long nb = n0 + cb * 4;
-    const bool nok = nb < N;
-    // v[r][c] raw element bits
-    uint32_t v[T::KG][4];
+  uint32_t v[T::KG][4];  // v[r][c] raw element bits
+  // Raw loads first, then the selects, so all of a thread's first-round loads are in
+  // flight together. Rows past K1 and groups past N read a clamped (valid) address and are
+  // zeroed afterwards. When B spans less than 4 GiB the addresses are 32-bit offsets
+  // from B (one vector register each).
+  using Raw = typename std::conditional<T::ESZ == 4, uint4,
+                                        typename std::conditional<T::ESZ == 2, uint2, uint32_t>::type>::type;
+  const bool off32 = (size_t)K1 * (size_t)N * T::ESZ < (1ull << 32);
+  // tasks per thread loaded before `issue` (BF16 tiles of 64 columns have two)
+  constexpr int FIRST = KIND == K_BF16 ? 2 : 1;
+  Raw x[FIRST][T::KG];
+  auto load = [&](int task, Raw(&x)[T::KG]) {
+    const int g = task / (TN / 4), cb = task % (TN / 4);
+    const long nb = n0 + cb * 4, k0 = (long)g * T::KG;
+    const long nbc = nb < N ? nb : 0;
+    if (off32) {
+      const unsigned n32 = (unsigned)N, c32 = (unsigned)nbc;
+#pragma unroll
+      for (int r = 0; r < T::KG; ++r) {
+        const unsigned kr = (unsigned)(k0 + r < K1 ? k0 + r : K1 - 1);
+        x[r] = *(const Raw *)(B + (kr * n32 + c32) * (unsigned)T::ESZ);
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < T::KG; ++r) {
+        const long kr = k0 + r < K1 ? k0 + r : K1 - 1;
+        x[r] = *(const Raw *)(B + ((size_t)kr * N + nbc) * T::ESZ);
+      }
+    }
+  };
+  auto unpack = [&](int task, const Raw(&x)[T::KG]) {
+    const int g = task / (TN / 4), cb = task % (TN / 4);
+    const long k0 = (long)g * T::KG;
+    const bool nok = n0 + cb * 4 < N;
 #pragma unroll
     for (int r = 0; r < T::KG; ++r) {
-      const long k = (long)g * T::KG + r;
-      const bool ok = nok && k < K1;
-      const char *p = B + ((size_t)(ok ? k : 0) * N + (ok ? nb : 0)) * T::ESZ;
-      if (T::ESZ == 4) {
-        uint4 x = *(const uint4 *)p;
-        v[r][0] = x.x; v[r][1] = x.y; v[r][2] = x.z; v[r][3] = x.w;
-      } else if (T::ESZ == 2) {
-        uint2 x = *(const uint2 *)p;
-        v[r][0] = x.x & 0xffffu; v[r][1] = x.x >> 16; v[r][2] = x.y & 0xffffu; v[r][3] = x.y >> 16;
+      const bool ok = nok && k0 + r < K1;
+      if constexpr (T::ESZ == 4) {
+        v[r][0] = ok ? x[r].x : 0u; v[r][1] = ok ? x[r].y : 0u; v[r][2] = ok ? x[r].z : 0u; v[r][3] = ok ? x[r].w : 0u;
+      } else if constexpr (T::ESZ == 2) {
+        const uint32_t a = ok ? x[r].x : 0u, b = ok ? x[r].y : 0u;
+        v[r][0] = a & 0xffffu; v[r][1] = a >> 16; v[r][2] = b & 0xffffu; v[r][3] = b >> 16;
       } else {
-        uint32_t x = *(const uint32_t *)p;
+        const uint32_t a = ok ? x[r] : 0u;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) v[r][c] = (x >> (8 * c)) & 0xffu;
+        for (int c = 0; c < 4; ++c) v[r][c] = (a >> (8 * c)) & 0xffu;
       }
-#pragma unroll
-      for (int c = 0; c < 4; ++c) v[r][c] = ok ? v[r][c] : 0u;
     }
+  };
+  auto write = [&](int task) {
+    const int g = task / (TN / 4), cb = task % (TN / 4);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      int n = cb * 4 + c;
-      int sl = cslot<TN>(g, n);
+      const int n = cb * 4 + c;
+      const int sl = cslot<TN>(g, n);
       if (KIND == K_F32) {
         arena[sl] = make_uint4(v[0][c], v[1][c], v[2][c], v[3][c]);
       } else if (KIND == K_BF16) {
@@ -244,10 +275,14 @@ __device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K
                                v[6][c] | (v[7][c] << 16));
       } else if (KIND == K_I8) {
         uint32_t w[4];
+        int cs = 0;
 #pragma unroll
-        for (int d = 0; d < 4; ++d)
+        for (int d = 0; d < 4; ++d) {
           w[d] = v[4 * d][c] | (v[4 * d + 1][c] << 8) | (v[4 * d + 2][c] << 16) | (v[4 * d + 3][c] << 24);
+          cs = __builtin_amdgcn_sdot4((int)w[d], 0x01010101, cs, false);
+        }
         arena[sl] = make_uint4(w[0], w[1], w[2], w[3]);
+        atomicAdd(&colsum[n], cs);
       } else {
         uint32_t h[4], l[4];
 #pragma unroll
@@ -257,41 +292,35 @@ __device__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K
         arena[lo_off + sl] = make_uint4(l[0], l[1], l[2], l[3]);
       }
     }
-  }
-}
-
-// INT8: colsum[n] = sum over the ngroups k-groups of X starting at group g0 of the
-// circular arena (SG groups) of s8 X[k][n].
-template <int TN>
-__device__ void colsum_lds(const uint4 *arena, int g0, int ngroups, int SG, int *colsum) {
-  // thread -> (column, slice of groups); reduce slices with LDS atomics (integer adds
-  // are order independent, so the result is deterministic).
+  };
   const int tid = fresh_tid();
-  for (int n = tid; n < TN; n += NTH) colsum[n] = 0;
-  __syncthreads();
-  constexpr int SL = NTH / TN;  // slices per column
-  int n = tid % TN, sl = tid / TN;
-  if (sl < SL) {
-    int s = 0;
-    for (int g = sl; g < ngroups; g += SL) {
-      const int ga = g0 + g < SG ? g0 + g : g0 + g - SG;
-      uint4 x = arena[cslot<TN>(ga, n)];
-      s = __builtin_amdgcn_sdot4((int)x.x, 0x01010101, s, false);
-      s = __builtin_amdgcn_sdot4((int)x.y, 0x01010101, s, false);
-      s = __builtin_amdgcn_sdot4((int)x.z, 0x01010101, s, false);
-      s = __builtin_amdgcn_sdot4((int)x.w, 0x01010101, s, false);
+#pragma unroll
+  for (int f = 0; f < FIRST; ++f)
+    if (tid + f * NTH < tasks) load(tid + f * NTH, x[f]);
+  __builtin_amdgcn_sched_barrier(0);
+  issue();
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int f = 0; f < FIRST; ++f)
+    if (tid + f * NTH < tasks) {
+      unpack(tid + f * NTH, x[f]);
+      write(tid + f * NTH);
     }
-    atomicAdd(&colsum[n], s);
+  for (int task = tid + FIRST * NTH; task < tasks; task += NTH) {
+    load(task, x[0]);
+    unpack(task, x[0]);
+    write(task);
   }
 }
 
 // Epilogue of one pass: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32 of
-// row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena.
+// row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena (INT8
+// also adds the s8 column sums of what it writes into csum: the next GEMM's bias).
 // Index math is 32-bit (arena rows and M fit easily) and the row-bound checks are
 // only evaluated in the block that straddles M.
 template <int KIND, int TNB>
 __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, long rb0_l, long n0, uint4 *arena,
-                                                const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
+                                                const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB], int *csum) {
   using T = CT<KIND>;
   constexpr int TN = TNB * 32;
   constexpr int RP = T::RP;
@@ -328,8 +357,10 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
     return;
   }
   // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q. Padded rows
-  // (m >= M) are stored as zero: they are the next GEMM's k-padding (and INT8 column
-  // sums include them).
+  // (m >= M) are stored as zero: they are the next GEMM's k-padding.
+  int cs[TNB];
+#pragma unroll
+  for (int j = 0; j < TNB; ++j) cs[j] = 0;
 #pragma unroll
   for (int i = 0; i < RP; ++i) {
     const int rbase = rb0 + i * 32;
@@ -350,6 +381,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
                                  (int)acc[i][j][4 * q + 3]);
           if (!full) w &= valid == 0 ? 0u : (0xffffffffu >> (8 * (4 - valid)));
           *((uint32_t *)dst + (pos >> 2)) = w;
+          cs[j] = __builtin_amdgcn_sdot4((int)w, 0x01010101, cs[j], false);
         } else {
           float v[4];
 #pragma unroll
@@ -370,6 +402,9 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
       }
     }
   }
+  if (KIND == K_I8)
+#pragma unroll
+    for (int j = 0; j < TNB; ++j) atomicAdd(&csum[j * 32 + l32], cs[j]);  // integer: order independent
 }
 
 // One workgroup runs the whole chain on columns [n0, n0 + TN) of B. P.A[i] are
@@ -397,6 +432,9 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   constexpr long CS = (long)PF * T::KC;      // rows of one chunk
   const int SG = P.S / T::KG;                // arena k-groups
   const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;  // lo plane (F32X3), in slots
+  // INT8: three buffers of TN column sums; GEMM i reads its bias from buffer i % 3,
+  // its epilogues sum Y_i into buffer (i + 1) % 3, and it clears buffer (i + 2) % 3
+  // for GEMM i + 1 (the barrier that ends GEMM i orders the clear before any use)
   int *colsum = (int *)(arena + (size_t)SG * TN * (KIND == K_F32X3 ? 2 : 1));
 
   const int lane = threadIdx.x & 63;
@@ -437,12 +475,29 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     }
   };
 
+  if (KIND == K_I8) {
+    for (int t = fresh_tid(); t < 2 * TN; t += NTH) colsum[t] = 0;
+    __syncthreads();
+  }
   // invariant: at the start of each pass fa[0] holds (or has in flight) chunk 0
   const char *apk = first_ptr(0);
-  if (apk) load_chunk(apk, fa[0]);
   const long K1c = (P.K[0] + CS - 1) / CS * CS;
-  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0);
+  // A wave with no rows in GEMM 0 loads (and ignores) wave 0's first chunk, which
+  // always exists: the loads stay unconditional and the wait counts after them exact.
+  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum,
+                    [&] { load_chunk(apk ? apk : (const char *)P.A[0], fa[0]); });
   __syncthreads();
+
+  // rows [Mp, Mc) of Y_gi (the k-padding of the next GEMM) must read as zero
+  auto zero_pad = [&](int gi) {
+    const long Mp = (P.M[gi] + 31) / 32 * 32, Mc = kspad_of<KIND>(P.M[gi]) * T::KC;
+    const int zg0 = (int)((P.ybase[gi] + Mp) / T::KG), nzg = (int)((Mc - Mp) / T::KG);
+    for (int t = fresh_tid(); t < nzg * TN; t += NTH) {
+      const int g = zg0 + t / TN < SG ? zg0 + t / TN : zg0 + t / TN - SG;
+      arena[cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
+      if (KIND == K_F32X3) arena[lo_off + cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
 
   for (int gi = 0; gi < P.num_gemm; ++gi) {
     const long M = P.M[gi], K = P.K[gi];
@@ -454,10 +509,6 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     const int kspad = (int)kspad_of<KIND>(K);
     const int nchunks = kspad / PF;
     const int ksteps = (int)((K + T::KC - 1) / T::KC);  // k-steps with real data
-    if (KIND == K_I8) {
-      colsum_lds<TN>(arena, xg0, (int)(kspad * T::KC / T::KG), SG, colsum);
-      __syncthreads();
-    }
     const size_t pass_stride = (size_t)NW * kspad * RP * FRAG;
     const char *next_gemm = last ? nullptr : first_ptr(gi + 1);
     auto chunk_g0 = [&](int c) {  // first arena group of chunk c of X_gi
@@ -468,7 +519,11 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     // so the MFMAs accumulate u8(a).b directly.
     int bias0[TNB];
 #pragma unroll
-    for (int j = 0; j < TNB; ++j) bias0[j] = (KIND == K_I8) ? 128 * colsum[j * 32 + (fresh_tid() & 31)] : 0;
+    for (int j = 0; j < TNB; ++j)
+      bias0[j] = (KIND == K_I8) ? 128 * colsum[(gi % 3) * TN + j * 32 + (fresh_tid() & 31)] : 0;
+    int *const csum_next = colsum + ((gi + 1) % 3) * TN;
+    if (KIND == K_I8)
+      for (int t = fresh_tid(); t < TN; t += NTH) colsum[((gi + 2) % 3) * TN + t] = 0;
 
     Acc acc[RP][TNB];
     long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
@@ -519,34 +574,41 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
         load_chunk(after, fa[0]);
       }
       apk = after;
-      if (final_pass && !last) held_rb0 = rb0;
-      else store_rows_impl<KIND, TNB>(P, gi, rb0, n0, arena, acc);
+      if (final_pass && !last && P.hold) held_rb0 = rb0;
+      else store_rows_impl<KIND, TNB>(P, gi, rb0, n0, arena, acc, csum_next);
     }
     if ((long)wave * RP * 32 >= Mp && next_gemm) {  // no rows here: fetch the next GEMM's first chunk
       apk = next_gemm;
       load_chunk(apk, fa[0]);
     }
-    __syncthreads();  // every wave is done reading X_gi
-    if (last) break;
-    if (held_rb0 >= 0) store_rows_impl<KIND, TNB>(P, gi, held_rb0, n0, arena, acc);
-    // rows [Mp, Mc) of Y_gi (the k-padding of the next GEMM) must read as zero
-    {
-      const long Mc = kspad_of<KIND>(M) * T::KC;
-      const int zg0 = (int)((P.ybase[gi] + Mp) / T::KG), nzg = (int)((Mc - Mp) / T::KG);
-      for (int t = fresh_tid(); t < nzg * TN; t += NTH) {
-        const int g = zg0 + t / TN < SG ? zg0 + t / TN : zg0 + t / TN - SG;
-        arena[cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
-        if (KIND == K_F32X3) arena[lo_off + cslot<TN>(g, t % TN)] = make_uint4(0u, 0u, 0u, 0u);
-      }
+    if (last) break;  // C is written; nothing in LDS is read again
+    if (!P.hold) {
+      // Y_gi does not meet X_gi: its padding rows are cleared now and one barrier
+      // both retires X_gi and publishes Y_gi
+      zero_pad(gi);
+      __syncthreads();
+      continue;
     }
+    __syncthreads();  // every wave is done reading X_gi
+    if (held_rb0 >= 0) store_rows_impl<KIND, TNB>(P, gi, held_rb0, n0, arena, acc, csum_next);
+    zero_pad(gi);
     __syncthreads();  // Y_gi complete
   }
+}
+
+// Workgroup b -> tile. Workgroups are dealt to the 8 XCDs round-robin (b % 8); this
+// gives each XCD a contiguous range of tiles, so neighbouring tiles that share cache
+// lines of B (64-column INT8 tiles use half a line per row) meet in one L2. A
+// bijection on [0, G) for any G.
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+  const int q = G / 8, r = G % 8, x = b % 8, i = b / 8;
+  return x * q + (x < r ? x : r) + i;
 }
 
 template <int KIND, int TNB>
 __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   extern __shared__ uint4 arena[];
-  chain_tile<KIND, TNB>(P, (long)blockIdx.x * (TNB * 32), arena);
+  chain_tile<KIND, TNB>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
 }
 
 constexpr int RP_of(int kind) {
@@ -565,26 +627,37 @@ constexpr int CS_of(int kind) {
                           : CT<K_F32X3>::PF * CT<K_F32X3>::KC;
 }
 
+// LDS bytes: S rows (both planes for F32X3) and the three INT8 column-sum buffers.
+size_t arena_bytes(int kind, int S, int tile_n) {
+  const size_t groups = (size_t)S / KG_of(kind);
+  return groups * tile_n * 16 * (kind == K_F32X3 ? 2 : 1) + (kind == K_I8 ? 3 * tile_n * sizeof(int) : 0);
+}
+
 // Circular arena of S rows: X_0 (the B tile) at row 0, Y_i right after X_i (mod S)
 // and X_{i+1} = Y_i; every region spans its k-extent rounded up to a chunk and starts
-// on a chunk boundary (S is a multiple of a chunk). All rows of Y_i but each wave's
-// final pass are written while X_i is still read, so they must not meet it:
-// Kc_i + (passes_i - 1) * PASS <= S.
-int plan_arena(int kind, const long *M, const long *K, int n, int *xbase, int *ybase) {
+// on a chunk boundary (S is a multiple of a chunk). When max_i (Kc_i + Mc_i) rows fit
+// the LDS at this tile width, Y_i never meets X_i and every pass is stored as soon as
+// it is done (*hold = 0). Otherwise the arena shrinks to what the held final passes
+// need (*hold = 1): all rows of Y_i but each wave's final pass are written while X_i
+// is still read, so they must not meet it: Kc_i + (passes_i - 1) * PASS <= S.
+int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *xbase, int *ybase, int *hold) {
   const long cs = CS_of(kind);
   auto rc = [cs](long v) { return (v + cs - 1) / cs * cs; };
   auto r32 = [](long v) { return (v + 31) / 32 * 32; };
   const long pass = (long)NW * RP_of(kind) * 32;
-  long S = rc(K[0]);
+  long S = rc(K[0]), wide = rc(K[0]);
   for (int i = 0; i < n; ++i) {
     const long kc = rc(K[i]);
     S = std::max(S, kc);
+    wide = std::max(wide, kc);
     if (i < n - 1) {
       const long passes = (r32(M[i]) + pass - 1) / pass;
       S = std::max(S, std::max(rc(M[i]), kc + (passes - 1) * pass));
+      wide = std::max(wide, kc + rc(M[i]));
     }
   }
-  S = rc(S);
+  *hold = tile_n <= 0 || arena_bytes(kind, (int)wide, tile_n) > LDS_LIMIT;
+  if (!*hold) S = wide;
   long x = 0;
   for (int i = 0; i < n; ++i) {
     xbase[i] = (int)x;
@@ -594,11 +667,6 @@ int plan_arena(int kind, const long *M, const long *K, int n, int *xbase, int *y
   return (int)S;
 }
 
-// LDS bytes: S rows (both planes for F32X3) and the INT8 column sums.
-size_t arena_bytes(int kind, int S, int tile_n) {
-  const size_t groups = (size_t)S / KG_of(kind);
-  return groups * tile_n * 16 * (kind == K_F32X3 ? 2 : 1) + (kind == K_I8 ? tile_n * sizeof(int) : 0);
-}
 
 template <int KIND, int TNB>
 hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
@@ -676,7 +744,8 @@ hipError_t launch_pack_t(PackJob J, hipStream_t s) {
 size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n) {
   int xb[MAXG], yb[MAXG];
   if (num_gemm < 1 || num_gemm > MAXG) return (size_t)-1;
-  const int S = plan_arena(kind, M, K, num_gemm, xb, yb);
+  int hold;
+  const int S = plan_arena(kind, M, K, num_gemm, tile_n, xb, yb, &hold);
   return arena_bytes(kind, S, tile_n);
 }
 
@@ -731,7 +800,7 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
     P->M[i] = c.M[i];
     P->K[i] = c.K[i];
   }
-  P->S = plan_arena(c.kind, c.M, c.K, c.num_gemm, P->xbase, P->ybase);
+  P->S = plan_arena(c.kind, c.M, c.K, c.num_gemm, c.tile_n, P->xbase, P->ybase, &P->hold);
   P->B = c.B;
   P->C = c.C;
   P->N = c.N;

@@ -267,6 +267,22 @@ def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
     return out
 
 
+def stream_calibration(fg, stream, clock_ghz=2.4):
+    """Per-CU rate of the fused chains' weight stream: every workgroup (one per CU)
+    streams 4 MB through 8 waves x 16 loads of 1 KB in flight from a 256 KB region all
+    workgroups share (L2 resident). Best of 5 launches per size."""
+    out = {}
+    slice_kb, rounds = 32, 255
+    per_wg = 8 * 16 * (rounds + 1) * 1024
+    for wgs in (256, 32):
+        ms = min(fg.measure_stream(slice_kb, rounds, wgs, stream=stream) for _ in range(5))
+        gbs = per_wg / (ms * 1e-3) / 1e9
+        out[f"workgroups_{wgs}"] = dict(ms=ms, bytes_per_workgroup=per_wg, gbs_per_cu=gbs,
+                                        bytes_per_clk_per_cu_at_2p4ghz=gbs / clock_ghz,
+                                        aggregate_gbs=gbs * wgs)
+    return out
+
+
 def cpu_baseline(inputs, budget_s=12.0):
     """The main workload on this host's cores: the compiled reference when it can
     run here, else the oracle's C restatement."""
@@ -314,6 +330,9 @@ def main() -> None:
     ap.add_argument("--main-only", action="store_true", help="skip the other configs and the CPU baseline")
     ap.add_argument("--only", default=None,
                     help="run just this named config (for profiling), print its result and exit")
+    ap.add_argument("--stream-calibration", action="store_true",
+                    help="measure the per-CU L2 -> CU stream rate of the fused chains' weight-stream pattern "
+                         "(fg_measure_stream) at 256 and 32 workgroups and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 5 device copies of 256 MiB (reads 256 MiB, writes 256 MiB each) and exit: "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
@@ -337,6 +356,9 @@ def main() -> None:
     fg.init(0)
     stream = torch.cuda.current_stream()
     timer = Timer(torch, stream)
+    if args.stream_calibration:
+        print(json.dumps({"stream_calibration": stream_calibration(fg, stream)}))
+        return
 
     extra = (
         ("sgemm_2048_fp32_mfma", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_MFMA)),

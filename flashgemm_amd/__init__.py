@@ -80,6 +80,7 @@ def lib() -> ctypes.CDLL:
         L.fg_f64_chain.argtypes = [vp, vp, l, i, vpp, lp, lp, vp]
         L.fg_to_f64.argtypes = [i, vp, vp, l, vp]
         L.fg_error_norms.argtypes = [i, vp, vp, l, ctypes.POINTER(ctypes.c_double), vp]
+        L.fg_measure_stream.argtypes = [l, i, i, ctypes.POINTER(ctypes.c_float), vp]
         L.fg_init.argtypes = [i]
         L.fg_synchronize.argtypes = [vp]
         L.fg_last_error.restype = ctypes.c_char_p
@@ -205,6 +206,14 @@ def error_norms(x_kind: int, x, ref, n: int, stream=None):
     out = (ctypes.c_double * 2)()
     _check(lib().fg_error_norms(x_kind, _addr(x), _addr(ref), n, out, _stream(stream)))
     return out[0], out[1]
+
+
+def measure_stream(slice_kb: int, rounds: int, workgroups: int, stream=None) -> float:
+    """Milliseconds for `workgroups` workgroups to stream 8 * 16 * (rounds + 1) KB each
+    (the fused chains' weight-stream pattern, measurement only)."""
+    ms = ctypes.c_float()
+    _check(lib().fg_measure_stream(slice_kb, rounds, workgroups, ctypes.byref(ms), _stream(stream)))
+    return ms.value
 
 
 def packed_weight_bytes(dtype: int, M: int, K: int, f32_mode: int = F32_MFMA) -> int:

@@ -499,6 +499,33 @@ int fg_error_norms(int x_kind, const void *x, const double *ref, long n, double 
   return hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
 }
 
+int fg_measure_stream(long slice_kb, int rounds, int workgroups, float *ms, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (slice_kb < 1 || slice_kb > 65536 || rounds < 1 || workgroups < 1 || workgroups > 65536 || !ms)
+    return set_error(FG_ERR_INVALID, "measure_stream slice_kb=%ld rounds=%d workgroups=%d", slice_kb, rounds,
+                     workgroups);
+  hipStream_t s = (hipStream_t)stream;
+  void *region, *out;
+  if ((rc = scratch(s, "stream_region", (size_t)8 * slice_kb * 1024, &region))) return rc;
+  if ((rc = scratch(s, "stream_out", (size_t)workgroups * 512 * sizeof(unsigned), &out))) return rc;
+  hipEvent_t e0, e1;
+  if ((rc = hip_status(hipEventCreate(&e0), "hipEventCreate"))) return rc;
+  if ((rc = hip_status(hipEventCreate(&e1), "hipEventCreate"))) {
+    (void)hipEventDestroy(e0);
+    return rc;
+  }
+  rc = hip_status(hipEventRecord(e0, s), "hipEventRecord");
+  if (!rc) rc = hip_status(launch_stream_probe(region, (unsigned)slice_kb, rounds, workgroups, (unsigned *)out, s),
+                           "stream probe");
+  if (!rc) rc = hip_status(hipEventRecord(e1, s), "hipEventRecord");
+  if (!rc) rc = hip_status(hipEventSynchronize(e1), "hipEventSynchronize");
+  if (!rc) rc = hip_status(hipEventElapsedTime(ms, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
 // ================================================================ part 1 ===
 
 int flashgemm_thread_num = 1;

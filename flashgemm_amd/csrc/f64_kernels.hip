@@ -1,6 +1,7 @@
-// FP64 oracle on the GPU: a plain FP64 GEMM and relative Frobenius error norms.
-// Used only to measure the error of the FP32 / BF16 / split-BF16 paths at full
-// size; never called on the product path.
+// Measurement kernels, never called on the product path: the FP64 oracle on the GPU
+// (a plain FP64 GEMM and relative Frobenius error norms, used to measure the error of
+// the FP32 / BF16 / split-BF16 paths at full size) and the L2 -> CU stream-rate
+// kernel behind the fused chains' bandwidth bound.
 #include "fg_internal.h"
 
 namespace fg {
@@ -104,7 +105,51 @@ __global__ void err_final_kernel(const double *part, double *out2) {
   out2[1] = b;
 }
 
+// The fused chains' weight stream on its own: wave w of every workgroup reads its
+// own contiguous slice of a region all workgroups share (8 slices), 16 B per lane,
+// 16 loads of 1 KB in flight per wave, `rounds` x 16 loads per wave. The loaded
+// values are folded into one word per lane so no load is dead.
+__global__ __launch_bounds__(512) void stream_kernel(const uint4 *__restrict__ region, unsigned slice_frags,
+                                                     int rounds, unsigned *__restrict__ out) {
+  constexpr int D = 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint4 *base = region + (size_t)wave * slice_frags * 64 + lane;
+  unsigned f = 0;
+  uint4 v[D], acc = make_uint4(0u, 0u, 0u, 0u);
+  // every component feeds its own accumulator, so each load stays one 16-byte load
+  auto fold = [&](const uint4 &x) {
+    acc.x ^= x.x;
+    acc.y += x.y;
+    acc.z ^= x.z;
+    acc.w += x.w;
+  };
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    v[d] = base[(size_t)f * 64];
+    f = f + 1 == slice_frags ? 0 : f + 1;
+  }
+  for (int it = 0; it < rounds; ++it) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      fold(v[d]);
+      v[d] = base[(size_t)f * 64];
+      f = f + 1 == slice_frags ? 0 : f + 1;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) fold(v[d]);
+  out[(size_t)blockIdx.x * 512 + threadIdx.x] = acc.x ^ acc.y ^ acc.z ^ acc.w;
+}
+
 }  // namespace
+
+// region: 8 * slice_frags KB, out: workgroups * 512 words.
+hipError_t launch_stream_probe(const void *region, unsigned slice_frags, int rounds, int workgroups, unsigned *out,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(stream_kernel, dim3((unsigned)workgroups), dim3(512), 0, s, (const uint4 *)region, slice_frags,
+                     rounds, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_f64_gemm(const double *A, const double *B, double *C, long M, long N, long K, hipStream_t s) {
   dim3 grid((unsigned)((N + T64 - 1) / T64), (unsigned)((M + T64 - 1) / T64));

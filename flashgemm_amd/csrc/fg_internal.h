@@ -233,5 +233,9 @@ hipError_t launch_f64_gemm(const double *A, const double *B, double *C, long M, 
 hipError_t launch_to_f64(int src_kind, const void *src, double *dst, long n, hipStream_t s);
 hipError_t launch_err_norms(int x_kind, const void *x, const double *ref, long n, double *out2,
                             hipStream_t s);
+// L2 -> CU stream-rate kernel (measurement): region of 8 * slice_frags KB, out of
+// workgroups * 512 words.
+hipError_t launch_stream_probe(const void *region, unsigned slice_frags, int rounds, int workgroups, unsigned *out,
+                               hipStream_t s);
 
 }  // namespace fg

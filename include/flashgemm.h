@@ -219,6 +219,14 @@ int fg_to_f64(int x_kind, const void *x, double *out, long n, void *stream);
  * out is a host pointer; the call synchronizes the stream. */
 int fg_error_norms(int x_kind, const void *x, const double *ref, long n, double *out, void *stream);
 
+/* ---- L2 -> CU stream rate (measurement only; no counterpart in the reference) ----
+ * `workgroups` workgroups of 8 waves; wave w of each streams its own slice_kb KB slice
+ * of a region the workgroups share (8 slices), 1 KB per load (16 B per lane), 16
+ * loads in flight, 16 * (rounds + 1) loads per wave: the access pattern of the fused
+ * chains' weight stream. *ms = elapsed time of the launch (HIP events on `stream`);
+ * bytes per workgroup = 8 * 16 * (rounds + 1) KB. */
+int fg_measure_stream(long slice_kb, int rounds, int workgroups, float *ms, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

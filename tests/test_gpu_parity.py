@@ -375,3 +375,11 @@ def test_int8_three_chain_full_size_properties(gpu):
     cols = np.sort(rng.choice(N, 96, replace=False))
     want = O.chain_int8(np.ascontiguousarray(B[:, cols]), As, O.LAYOUT_MATH)
     assert np.array_equal(fused[:, cols], want)
+
+
+def test_measure_stream_entry_point(gpu):
+    """The stream-rate measurement runs (a positive time) and rejects bad sizes."""
+    assert fg.measure_stream(32, 3, 8) > 0
+    with pytest.raises(fg.FlashgemmError) as e:
+        fg.measure_stream(0, 3, 8)
+    assert e.value.status == fg.ERR_INVALID

@@ -213,11 +213,11 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
   // one task = KG rows x 4 columns of one k-group
   const int ngroups = (int)(K1p / T::KG);
   const int tasks = ngroups * (TN / 4);
-  uint32_t v[T::KG][4];  // v[r][c] raw element bits
-  // Raw loads first, then the selects, so all of a thread's first-round loads are in
-  // flight together. Rows past K1 and groups past N read a clamped (valid) address and are
-  // zeroed afterwards. When B spans less than 4 GiB the addresses are 32-bit offsets
-  // from B (one vector register each).
+  // Raw loads first, then the transposition into slots, so all of a thread's
+  // first-round loads are in flight together. Rows past K1 and groups past N read a
+  // clamped (valid) address and are zeroed afterwards. When B spans less than 4 GiB
+  // the addresses are 32-bit offsets from B, and a group inside B (every task but
+  // those at the K1 edge) adds a uniform row stride per row.
   using Raw = typename std::conditional<T::ESZ == 4, uint4,
                                         typename std::conditional<T::ESZ == 2, uint2, uint32_t>::type>::type;
   const bool off32 = (size_t)K1 * (size_t)N * T::ESZ < (1ull << 32);
@@ -229,11 +229,17 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
     const long nb = n0 + cb * 4, k0 = (long)g * T::KG;
     const long nbc = nb < N ? nb : 0;
     if (off32) {
-      const unsigned n32 = (unsigned)N, c32 = (unsigned)nbc;
+      const unsigned rb = (unsigned)N * T::ESZ, c32 = (unsigned)nbc * T::ESZ;
+      if (k0 + T::KG <= K1) {
+        const unsigned o0 = (unsigned)k0 * rb + c32;
 #pragma unroll
-      for (int r = 0; r < T::KG; ++r) {
-        const unsigned kr = (unsigned)(k0 + r < K1 ? k0 + r : K1 - 1);
-        x[r] = *(const Raw *)(B + (kr * n32 + c32) * (unsigned)T::ESZ);
+        for (int r = 0; r < T::KG; ++r) x[r] = *(const Raw *)(B + (o0 + (unsigned)r * rb));
+      } else {
+#pragma unroll
+        for (int r = 0; r < T::KG; ++r) {
+          const unsigned kr = (unsigned)(k0 + r < K1 ? k0 + r : K1 - 1);
+          x[r] = *(const Raw *)(B + (kr * rb + c32));
+        }
       }
     } else {
 #pragma unroll
@@ -243,53 +249,64 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
       }
     }
   };
-  auto unpack = [&](int task, const Raw(&x)[T::KG]) {
+  // element (row r, column c) of a task as raw bits: component c of x[r] (FP32)
+  auto comp = [](const uint4 &v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); };
+  // Transpose a task (KG rows x 4 columns) into its 4 slots: byte permutes for INT8
+  // (a 4 x 4 byte transpose is 8 v_perm_b32) and BF16 (one v_perm_b32 per pair of rows).
+  auto put = [&](int task, Raw(&x)[T::KG]) {
     const int g = task / (TN / 4), cb = task % (TN / 4);
     const long k0 = (long)g * T::KG;
     const bool nok = n0 + cb * 4 < N;
+    if (!(nok && k0 + T::KG <= K1)) {  // edge task
 #pragma unroll
-    for (int r = 0; r < T::KG; ++r) {
-      const bool ok = nok && k0 + r < K1;
-      if constexpr (T::ESZ == 4) {
-        v[r][0] = ok ? x[r].x : 0u; v[r][1] = ok ? x[r].y : 0u; v[r][2] = ok ? x[r].z : 0u; v[r][3] = ok ? x[r].w : 0u;
-      } else if constexpr (T::ESZ == 2) {
-        const uint32_t a = ok ? x[r].x : 0u, b = ok ? x[r].y : 0u;
-        v[r][0] = a & 0xffffu; v[r][1] = a >> 16; v[r][2] = b & 0xffffu; v[r][3] = b >> 16;
-      } else {
-        const uint32_t a = ok ? x[r] : 0u;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[r][c] = (a >> (8 * c)) & 0xffu;
-      }
+      for (int r = 0; r < T::KG; ++r)
+        if (!(nok && k0 + r < K1)) x[r] = Raw{};
     }
-  };
-  auto write = [&](int task) {
-    const int g = task / (TN / 4), cb = task % (TN / 4);
+    uint4 *const sl = arena + cslot<TN>(g, cb * 4);
+    if constexpr (KIND == K_F32) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int n = cb * 4 + c;
-      const int sl = cslot<TN>(g, n);
-      if (KIND == K_F32) {
-        arena[sl] = make_uint4(v[0][c], v[1][c], v[2][c], v[3][c]);
-      } else if (KIND == K_BF16) {
-        arena[sl] = make_uint4(v[0][c] | (v[1][c] << 16), v[2][c] | (v[3][c] << 16), v[4][c] | (v[5][c] << 16),
-                               v[6][c] | (v[7][c] << 16));
-      } else if (KIND == K_I8) {
+      for (int c = 0; c < 4; ++c) sl[c] = make_uint4(comp(x[0], c), comp(x[1], c), comp(x[2], c), comp(x[3], c));
+    } else if constexpr (KIND == K_BF16) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const uint32_t sel = (c & 1) ? 0x07060302u : 0x05040100u;  // high or low halves of both words
         uint32_t w[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          w[e] = __builtin_amdgcn_perm(c < 2 ? x[2 * e + 1].x : x[2 * e + 1].y, c < 2 ? x[2 * e].x : x[2 * e].y, sel);
+        sl[c] = make_uint4(w[0], w[1], w[2], w[3]);
+      }
+    } else if constexpr (KIND == K_I8) {
+      uint32_t w[4][4];  // w[c][d]: rows 4d..4d+3 of column c
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const uint32_t t0 = __builtin_amdgcn_perm(x[4 * d + 1], x[4 * d], 0x05010400u);      // r0.b0 r1.b0 r0.b1 r1.b1
+        const uint32_t t1 = __builtin_amdgcn_perm(x[4 * d + 3], x[4 * d + 2], 0x05010400u);  // r2.b0 r3.b0 r2.b1 r3.b1
+        const uint32_t t2 = __builtin_amdgcn_perm(x[4 * d + 1], x[4 * d], 0x07030602u);      // r0.b2 r1.b2 r0.b3 r1.b3
+        const uint32_t t3 = __builtin_amdgcn_perm(x[4 * d + 3], x[4 * d + 2], 0x07030602u);
+        w[0][d] = __builtin_amdgcn_perm(t1, t0, 0x05040100u);
+        w[1][d] = __builtin_amdgcn_perm(t1, t0, 0x07060302u);
+        w[2][d] = __builtin_amdgcn_perm(t3, t2, 0x05040100u);
+        w[3][d] = __builtin_amdgcn_perm(t3, t2, 0x07060302u);
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
         int cs = 0;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          w[d] = v[4 * d][c] | (v[4 * d + 1][c] << 8) | (v[4 * d + 2][c] << 16) | (v[4 * d + 3][c] << 24);
-          cs = __builtin_amdgcn_sdot4((int)w[d], 0x01010101, cs, false);
-        }
-        arena[sl] = make_uint4(w[0], w[1], w[2], w[3]);
-        atomicAdd(&colsum[n], cs);
-      } else {
+        for (int d = 0; d < 4; ++d) cs = __builtin_amdgcn_sdot4((int)w[c][d], 0x01010101, cs, false);
+        sl[c] = make_uint4(w[c][0], w[c][1], w[c][2], w[c][3]);
+        atomicAdd(&colsum[cb * 4 + c], cs);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
         uint32_t h[4], l[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          split_pk(__builtin_bit_cast(float, v[2 * r][c]), __builtin_bit_cast(float, v[2 * r + 1][c]), h[r], l[r]);
-        arena[sl] = make_uint4(h[0], h[1], h[2], h[3]);
-        arena[lo_off + sl] = make_uint4(l[0], l[1], l[2], l[3]);
+          split_pk(__builtin_bit_cast(float, comp(x[2 * r], c)), __builtin_bit_cast(float, comp(x[2 * r + 1], c)), h[r],
+                   l[r]);
+        sl[c] = make_uint4(h[0], h[1], h[2], h[3]);
+        sl[lo_off + c] = make_uint4(l[0], l[1], l[2], l[3]);
       }
     }
   };
@@ -302,14 +319,10 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int f = 0; f < FIRST; ++f)
-    if (tid + f * NTH < tasks) {
-      unpack(tid + f * NTH, x[f]);
-      write(tid + f * NTH);
-    }
+    if (tid + f * NTH < tasks) put(tid + f * NTH, x[f]);
   for (int task = tid + FIRST * NTH; task < tasks; task += NTH) {
     load(task, x[0]);
-    unpack(task, x[0]);
-    write(task);
+    put(task, x[0]);
   }
 }
 

@@ -277,10 +277,6 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
                     hipStream_t s) {
   int *colsum = nullptr;
   int rc;
-  if (kind == K_I8) {
-    if ((rc = scratch(s, "colsum", sizeof(int) * N, (void **)&colsum))) return rc;
-    if ((rc = hip_status(launch_colsum_i8((const int8_t *)B, K, N, colsum, s), "colsum"))) return rc;
-  }
   if (kind == K_F32 && epi != EPI_PARTIAL && splits <= 0 && skinny_f32_shape(M, N)) {
     // small M or N: 64 x 64 tiles, split-K over the grid
     long kper = 0;
@@ -294,9 +290,13 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
     return hip_status(launch_splitk_reduce(kind, EPI_STORE, out, sk, M, N, nullptr, C, s), "split-K reduce");
   }
   if (splits <= 0) splits = plan_splits(M, N, K, kind);
-  if (splits <= 1) {
-    GemmArgs g{kind, epi, A, B, C, colsum, M, N, K, 1};
+  if (splits <= 1) {  // INT8: the kernel sums its own B columns (colsum == nullptr)
+    GemmArgs g{kind, epi, A, B, C, nullptr, M, N, K, 1};
     return hip_status(launch_gemm(g, s), "gemm");
+  }
+  if (kind == K_I8) {  // split-K slabs see part of K: the reduction adds the full column sums
+    if ((rc = scratch(s, "colsum", sizeof(int) * N, (void **)&colsum))) return rc;
+    if ((rc = hip_status(launch_colsum_i8((const int8_t *)B, K, N, colsum, s), "colsum"))) return rc;
   }
   void *part = nullptr;
   if ((rc = scratch(s, "partials", (size_t)splits * M * N * 4, &part))) return rc;

@@ -138,7 +138,7 @@ struct GemmArgs {
   const void *A;    // M x K row-major (int8 read as u8, bf16 raw bits)
   const void *B;    // K x N row-major (int8 read as s8)
   void *C;          // M x N row-major, element type by (kind, epi)
-  const int *colsum;// INT8 only: sum_k B[k][n] for the same K range (int32, N entries)
+  const int *colsum;// INT8 split-K: sum_k B[k][n] (int32, N entries); nullptr: the kernel sums its panel
   long M, N, K;
   int splits;       // split-K factor (EPI_PARTIAL writes splits slabs of M x N)
 };

@@ -201,30 +201,43 @@ struct Stager {
     }
   }
 
-  __device__ __forceinline__ void store_b(uint4 *sB, int t) const {
+  // Rows of B become k-groups of a column: a 4 x 4 byte transpose per 4 rows (8
+  // v_perm_b32) for INT8, one v_perm_b32 per pair of rows for BF16.
+  // INT8 also adds the s8 sums of the 4 columns it stages into cs (the column sums of
+  // the workgroup's B panel, for the u8 bias).
+  __device__ __forceinline__ void store_b(uint4 *sB, int t, int (&cs)[4]) const {
     constexpr int NCB = BT / T::CW;
     int cb = t % NCB, g = t / NCB;
+    uint32_t wi[4][4];  // INT8: wi[c][d] = rows 4d..4d+3 of column c
+    if (KIND == K_I8) {
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const uint32_t t0 = __builtin_amdgcn_perm(b[4 * d + 1], b[4 * d], 0x05010400u);
+        const uint32_t t1 = __builtin_amdgcn_perm(b[4 * d + 3], b[4 * d + 2], 0x05010400u);
+        const uint32_t t2 = __builtin_amdgcn_perm(b[4 * d + 1], b[4 * d], 0x07030602u);
+        const uint32_t t3 = __builtin_amdgcn_perm(b[4 * d + 3], b[4 * d + 2], 0x07030602u);
+        wi[0][d] = __builtin_amdgcn_perm(t1, t0, 0x05040100u);
+        wi[1][d] = __builtin_amdgcn_perm(t1, t0, 0x07060302u);
+        wi[2][d] = __builtin_amdgcn_perm(t3, t2, 0x05040100u);
+        wi[3][d] = __builtin_amdgcn_perm(t3, t2, 0x07060302u);
+      }
+    }
 #pragma unroll
     for (int c = 0; c < T::CW; ++c) {
       int n = cb * T::CW + c;
       if (KIND == K_F32) {
         sB[slot<BT>(g, n)] = make_uint4(b[c], b[4 + c], b[8 + c], b[12 + c]);
       } else if (KIND == K_BF16) {
+        const uint32_t sel = (c & 1) ? 0x07060302u : 0x05040100u;  // high or low halves of both words
         uint32_t w[4];
-        for (int d = 0; d < 4; ++d) {
-          uint32_t lo = (b[2 * (2 * d) + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
-          uint32_t hi = (b[2 * (2 * d + 1) + (c >> 1)] >> (16 * (c & 1))) & 0xffffu;
-          w[d] = lo | (hi << 16);
-        }
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+          w[d] = __builtin_amdgcn_perm(b[2 * (2 * d + 1) + (c >> 1)], b[2 * (2 * d) + (c >> 1)], sel);
         sB[slot<BT>(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
       } else if (KIND == K_I8) {
-        uint32_t w[4];
-        for (int d = 0; d < 4; ++d) {
-          uint32_t x = 0u;
-          for (int e = 0; e < 4; ++e) x |= ((b[4 * d + e] >> (8 * c)) & 0xffu) << (8 * e);
-          w[d] = x;
-        }
-        sB[slot<BT>(g, n)] = make_uint4(w[0], w[1], w[2], w[3]);
+        sB[slot<BT>(g, n)] = make_uint4(wi[c][0], wi[c][1], wi[c][2], wi[c][3]);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) cs[c] = __builtin_amdgcn_sdot4((int)wi[c][d], 0x01010101, cs[c], false);
       } else {  // F32X3: 8 floats of column c -> hi and lo planes
         uint32_t h[4], l[4];
         for (int r = 0; r < 4; ++r)
@@ -337,12 +350,13 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
   const char *A = (const char *)Av;
   const char *B = (const char *)Bv;
   Stager<KIND, TL> st;
+  int cs[4] = {0, 0, 0, 0};
   int nstages = (int)((kend - kbeg + T::BK - 1) / T::BK);
   if (nstages > 0) {
     st.load_a(A, M, K, kend, m0, kbeg, vecA, t);
     st.load_b(B, N, kend, n0, kbeg, vecB, t);
     st.store_a(smem, t);
-    st.store_b(smem + TILE_U4, t);
+    st.store_b(smem + TILE_U4, t, cs);
   }
   __syncthreads();
   for (int s = 0; s < nstages; ++s) {
@@ -358,9 +372,29 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
     for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND, TL>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
     if (more) {
       st.store_a(nxt, t);
-      st.store_b(nxt + TILE_U4, t);
+      st.store_b(nxt + TILE_U4, t, cs);
     }
     __syncthreads();
+  }
+  // INT8 without a column-sum array (no split-K): the workgroup staged every row of its
+  // B panel, so it sums the panel's columns itself (threads of one column block added
+  // in LDS, integers: exact in any order).
+  const int *csum_tile = nullptr;
+  if (KIND == K_I8 && EPI != EPI_PARTIAL && colsum == nullptr) {
+    constexpr int NCB = BT / KT<K_I8>::CW, NG = TL::NT / NCB;
+    int *red = (int *)smem;  // NG x BT partial sums, then BT totals
+    const int cb = t % NCB, g = t / NCB;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) red[g * BT + cb * 4 + c] = cs[c];
+    __syncthreads();
+    for (int n = t; n < BT; n += TL::NT) {
+      int sum = 0;
+#pragma unroll
+      for (int q = 0; q < NG; ++q) sum += red[q * BT + n];
+      red[NG * BT + n] = sum;
+    }
+    __syncthreads();
+    csum_tile = red + NG * BT;
   }
 
   // ---- epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column lane%32 per block
@@ -370,7 +404,7 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
     long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
     if (n >= N) continue;
     int bias = 0;
-    if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * colsum[n];
+    if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
 #pragma unroll
     for (int i = 0; i < IM; ++i)
 #pragma unroll

@@ -399,6 +399,41 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
 
   // ---- epilogue: lane holds rows 8(r/4) + 4h + (r%4) of column lane%32 per block
   const int h = lane >> 5, l32 = lane & 31;
+  // bytes per element of C: partial slabs and the INT8 STORE are 32-bit, REQUANT
+  // writes the next operand (int8, bf16, fp32)
+  constexpr int CE = (EPI == EPI_REQUANT && KIND == K_I8) ? 1 : ((EPI == EPI_REQUANT && KIND == K_BF16) ? 2 : 4);
+  auto put = [&](char *p, const typename KT<KIND>::Acc &a, int r, int bias) {
+    if (EPI == EPI_PARTIAL) {
+      if (KIND == K_I8) *(int *)p = (int)a[r];
+      else *(float *)p = (float)a[r];
+    } else if (KIND == K_I8) {
+      const int v = (int)((unsigned)a[r] + (unsigned)bias);
+      if (EPI == EPI_STORE) *(int *)p = v;
+      else *(int8_t *)p = sat_i8(v);
+    } else if (KIND == K_BF16 && EPI == EPI_REQUANT) {
+      *(uint16_t *)p = bf16_dev((float)a[r]);
+    } else {
+      *(float *)p = (float)a[r];
+    }
+  };
+  char *const Cb = (char *)Cv + (EPI == EPI_PARTIAL ? (size_t)blockIdx.y * M * N * CE : 0);
+  if (m0 + BT <= M && n0 + BT <= N) {
+    // whole tile inside C: one lane pointer per block, rows at uniform strides
+    const size_t rs = (size_t)N * CE;
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+      const long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
+      int bias = 0;
+      if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
+#pragma unroll
+      for (int i = 0; i < IM; ++i) {
+        char *const p = Cb + ((size_t)(m0 + wm * (BT / TL::WM) + i * 32 + 4 * h) * N + n) * CE;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) put(p + (size_t)(8 * (r >> 2) + (r & 3)) * rs, acc[i][j], r, bias);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < JN; ++j) {
     long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
@@ -411,20 +446,7 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
       for (int r = 0; r < 16; ++r) {
         long m = m0 + wm * (BT / TL::WM) + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
         if (m >= M) continue;
-        size_t off = (size_t)m * N + n;
-        if (EPI == EPI_PARTIAL) {
-          off += (size_t)blockIdx.y * M * N;
-          if (KIND == K_I8) ((int *)Cv)[off] = (int)acc[i][j][r];
-          else ((float *)Cv)[off] = (float)acc[i][j][r];
-        } else if (KIND == K_I8) {
-          int v = (int)((unsigned)acc[i][j][r] + (unsigned)bias);
-          if (EPI == EPI_STORE) ((int *)Cv)[off] = v;
-          else ((int8_t *)Cv)[off] = sat_i8(v);
-        } else if (KIND == K_BF16 && EPI == EPI_REQUANT) {
-          ((uint16_t *)Cv)[off] = bf16_dev((float)acc[i][j][r]);
-        } else {
-          ((float *)Cv)[off] = (float)acc[i][j][r];
-        }
+        put(Cb + ((size_t)m * N + n) * CE, acc[i][j], r, bias);
       }
   }
 }

@@ -58,8 +58,17 @@ template <> struct KT<K_F32X3> {
 
 // Operand tile per stage: BK/KG groups x BT rows x 16 B, or for F32X3 two planes
 // (hi, lo) of BK/8 groups x BT rows x 16 B: 8 x BT 16-byte slots either way.
+// Row r of group g sits at g*BT + (f(r) ^ (g & 7)) with
+// f(r) = r ^ 4(r & 1) ^ ((r >> 3) & 1) ^ 2((r >> 4) & 1), a permutation within every
+// aligned 32 rows chosen so that eight 16-byte slots (one LDS cycle of 128 bytes) land
+// on eight different bank groups for every access pattern here: 8 consecutive rows of
+// one group (fragment reads), one row in 8 groups (A staging), rows r and r+1 in 4
+// groups (split-bf16 A staging, 8-byte halves), every second and every fourth column
+// of one group (B staging with 2 or 4 columns per thread).
 template <int BT>
-__device__ __forceinline__ int slot(int g, int r) { return g * BT + (r ^ (g & 7)); }
+__device__ __forceinline__ int slot(int g, int r) {
+  return g * BT + ((r ^ ((r & 1) << 2) ^ ((r >> 3) & 1) ^ (((r >> 4) & 1) << 1)) ^ (g & 7));
+}
 
 // ---------------------------------------------------------------- staging --
 template <int KIND, class TL>

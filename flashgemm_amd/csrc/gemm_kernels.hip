@@ -788,7 +788,7 @@ bool skinny_f32_shape(long M, long N) { return M <= SK_TILE || N <= SK_TILE; }
 
 int skinny_f32_splits(long M, long N, long K, long *kper_out) {
   const long tiles = ((M + SK_TILE - 1) / SK_TILE) * ((N + SK_TILE - 1) / SK_TILE);
-  long want = (512 + tiles - 1) / tiles;  // about two workgroups per CU
+  long want = (256 + tiles - 1) / tiles;  // about one workgroup per CU (measured faster than two)
   long kper = (K + want - 1) / want;
   kper = (kper + 4 * SK_GROUP - 1) / (4 * SK_GROUP) * (4 * SK_GROUP);
   if (kper < 8 * SK_GROUP) kper = 8 * SK_GROUP;  // at least two groups per wave

@@ -181,6 +181,44 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
     return out
 
 
+def bench_conv(torch, fg, timer, stream, dtype, channels, size, ksize, Ms, steps=20, warmup=3):
+    """Convolution-shaped chain of the reference's conv tests (multigemm/test/test_int8_3.c:13-48,
+    test_f32_3.c:14-48): GEMM 1 is a ksize x ksize convolution (pad ksize // 2) of a
+    channels x size x size image, K1 = channels * ksize^2, N = size^2; then 1x1 layers.
+    Fused gather (fg_chain_conv: the chain kernel reads patches from the image) against
+    forming B first (fg_im2col + fg_chain on the patch matrix)."""
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    g = fg.conv_geom(1, channels, size, size, ksize, ksize, pad=ksize // 2)
+    K1, N = fg.conv_dims(g)
+    Ks = [K1] + Ms[:-1]
+    if dtype == fg.DTYPE_INT8:
+        X = torch.randint(-127, 128, (1, channels, size, size), generator=gen, device="cuda", dtype=torch.int8)
+        As = [torch.randint(-127, 128, (m, k), generator=gen, device="cuda", dtype=torch.int8) for m, k in zip(Ms, Ks)]
+        C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.int32)
+        e = 1
+    else:
+        X = torch.randn((1, channels, size, size), generator=gen, device="cuda").to(torch.bfloat16)
+        As = [(torch.randn((m, k), generator=gen, device="cuda") / k ** 0.5).to(torch.bfloat16) for m, k in zip(Ms, Ks)]
+        C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float32)
+        e = 2
+    B = torch.empty((K1, N), device="cuda", dtype=X.dtype)
+    ms = timer.run(lambda: fg.chain_conv(dtype, C, X, g, As, stream=stream), steps, warmup)
+    Cf = torch.empty_like(C)
+
+    def formed():
+        fg.im2col(dtype, B, X, g, stream=stream)
+        fg.chain(dtype, Cf, B, As, stream=stream)
+    ms_formed = timer.run(formed, steps, warmup)
+    torch.cuda.synchronize()
+    flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
+    peak = PEAK["int8" if dtype == fg.DTYPE_INT8 else "bf16"]
+    byts = e * (channels * size * size + sum(m * k for m, k in zip(Ms, Ks))) + 4 * Ms[-1] * N
+    return dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / peak, ms_formed_b=ms_formed,
+                gather_speedup=ms_formed / ms, same_bits_as_formed_b=bool(torch.equal(C, Cf)),
+                K1=K1, N=N, algorithmic_bytes=byts, b_bytes_not_formed=e * K1 * N,
+                fused=fg.chain_plan(dtype, (N + 3) // 4 * 4, Ms, Ks)[0])
+
+
 def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0), seed=5):
     """10,000 chain requests with Poisson arrivals at a fixed rate against 64 resident
     weight sets: chain length uniform in 2..4, widths from {64 .. 2048}, dtype mix 50%
@@ -367,6 +405,12 @@ def main() -> None:
         ("sgemm_64x64x65536", lambda: bench_sgemm(torch, fg, timer, stream, 64, 64, 65536, fg.F32_MFMA)),
         ("chain3_int8_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_INT8)),
         ("chain3_bf16_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_BF16)),
+        ("conv3x3_c32_256_int8", lambda: bench_conv(torch, fg, timer, stream, fg.DTYPE_INT8, 32, 256, 3,
+                                                     [32, 24, 144])),
+        ("conv3x3_c32_256_bf16", lambda: bench_conv(torch, fg, timer, stream, fg.DTYPE_BF16, 32, 256, 3,
+                                                     [32, 24, 144])),
+        ("conv5x5_c192_64_int8", lambda: bench_conv(torch, fg, timer, stream, fg.DTYPE_INT8, 192, 64, 5,
+                                                     [192, 56, 336])),
         ("precision_fp32_4chain", lambda: bench_precision(torch, fg, timer, stream)),
         ("serving_10k", lambda: bench_serving(torch, fg)),
     )

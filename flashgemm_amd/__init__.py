@@ -53,6 +53,21 @@ class ChainReq(ctypes.Structure):
                 ("a_packed", ctypes.c_int)]
 
 
+class ConvGeom(ctypes.Structure):
+    """fg_conv_geom: an NCHW image and the patches a convolution reads from it."""
+    _fields_ = [(f, ctypes.c_long) for f in ("batch", "channels", "height", "width", "kernel_h", "kernel_w",
+                                             "stride_h", "stride_w", "pad_h", "pad_w", "dilation_h",
+                                             "dilation_w")]
+
+
+def conv_geom(batch: int, channels: int, height: int, width: int, kernel_h: int, kernel_w: int, stride: int = 1,
+              pad: int = 0, dilation: int = 1, stride_w: int | None = None, pad_w: int | None = None,
+              dilation_w: int | None = None) -> ConvGeom:
+    return ConvGeom(batch, channels, height, width, kernel_h, kernel_w, stride,
+                    stride if stride_w is None else stride_w, pad, pad if pad_w is None else pad_w, dilation,
+                    dilation if dilation_w is None else dilation_w)
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
                 ("num_streams", ctypes.c_int), ("max_batch", ctypes.c_int), ("use_graph", ctypes.c_int),
@@ -75,6 +90,10 @@ def lib() -> ctypes.CDLL:
         L.fg_chain_host.argtypes = [i, vp, vp, l, i, vpp, lp, lp, ctypes.POINTER(ChainOpts)]
         L.fg_chain_plan.argtypes = [i, l, i, lp, lp, ctypes.POINTER(ChainOpts), ctypes.POINTER(i),
                                     ctypes.POINTER(i)]
+        L.fg_conv_dims.argtypes = [ctypes.POINTER(ConvGeom), lp, lp]
+        L.fg_im2col.argtypes = [i, vp, vp, ctypes.POINTER(ConvGeom), vp]
+        L.fg_chain_conv.argtypes = [i, vp, vp, ctypes.POINTER(ConvGeom), i, vpp, lp, lp,
+                                    ctypes.POINTER(ChainOpts), vp]
         L.fg_sgemm.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_host.argtypes = [vp, vp, vp, l, l, l, i]
         L.fg_f64_chain.argtypes = [vp, vp, l, i, vpp, lp, lp, vp]
@@ -185,6 +204,26 @@ def chain_plan(dtype: int, N: int, Ms, Ks, o: ChainOpts | None = None):
     _check(lib().fg_chain_plan(dtype, N, n, (ctypes.c_long * n)(*Ms), (ctypes.c_long * n)(*Ks),
                                ctypes.byref(o) if o is not None else None, ctypes.byref(fused), ctypes.byref(tn)))
     return bool(fused.value), tn.value
+
+
+def conv_dims(g: ConvGeom):
+    """(K1, N) of the patch matrix of geometry g."""
+    K1, N = ctypes.c_long(), ctypes.c_long()
+    _check(lib().fg_conv_dims(ctypes.byref(g), ctypes.byref(K1), ctypes.byref(N)))
+    return K1.value, N.value
+
+
+def im2col(dtype: int, B, X, g: ConvGeom, stream=None) -> None:
+    """B (K1 x N) = patch matrix of the NCHW image X on device buffers."""
+    _check(lib().fg_im2col(dtype, _addr(B), _addr(X), ctypes.byref(g), _stream(stream)))
+
+
+def chain_conv(dtype: int, C, X, g: ConvGeom, As, Ms=None, Ks=None, o: ChainOpts | None = None,
+               stream=None) -> None:
+    """C = A_n ... A_1 im2col(X) on device buffers; the fused chain gathers its B tile from X."""
+    n, ptrs, M, K = _arrays(As, Ms, Ks)
+    _check(lib().fg_chain_conv(dtype, _addr(C), _addr(X), ctypes.byref(g), n, ptrs, M, K,
+                               ctypes.byref(o) if o is not None else None, _stream(stream)))
 
 
 def sgemm(C, A, B, M: int, N: int, K: int, f32_mode: int = F32_MFMA, splits: int = 0, stream=None) -> None:

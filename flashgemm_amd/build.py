@@ -21,6 +21,7 @@ SOURCES = [
     "gemm_kernels.hip",
     "chain_kernel.hip",
     "f64_kernels.hip",
+    "im2col.hip",
     "grouped.cpp",
     "api.cpp",
     "serving.cpp",

@@ -309,6 +309,50 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
   return hip_status(launch_splitk_reduce(kind, epi, part, used, M, N, colsum, C, s), "split-K reduce");
 }
 
+// Arguments of a fused launch (A[i] as given; run_fused packs them unless a_packed).
+static ChainArgs chain_args(int kind, const void *B, void *C, long N, int num_gemm, const void *const *A,
+                            const long *M, const long *K, int tile_n) {
+  ChainArgs c{};
+  c.kind = kind;
+  c.num_gemm = num_gemm;
+  for (int i = 0; i < num_gemm; ++i) {
+    c.A[i] = A[i];
+    c.M[i] = M[i];
+    c.K[i] = K[i];
+  }
+  c.B = B;
+  c.C = C;
+  c.N = N;
+  c.tile_n = tile_n;
+  return c;
+}
+
+// Fused launch: unless the weights come packed, pack every A_i into MFMA fragment
+// order first (per-stream scratch, one launch for the whole chain).
+static int run_fused(ChainArgs c, int a_packed, hipStream_t s) {
+  int rc;
+  if (!a_packed) {
+    size_t off[CHAIN_MAXG], total = 0;
+    for (int i = 0; i < c.num_gemm; ++i) {
+      off[i] = total;
+      total += (packed_a_bytes(c.kind, c.M[i], c.K[i]) + 255) & ~(size_t)255;
+    }
+    char *pk = nullptr;
+    if ((rc = scratch(s, "apack", total, (void **)&pk))) return rc;
+    PackJob J{};
+    J.n = c.num_gemm;
+    for (int i = 0; i < c.num_gemm; ++i) {
+      J.A[i] = c.A[i];
+      J.M[i] = c.M[i];
+      J.K[i] = c.K[i];
+      J.out[i] = pk + off[i];
+      c.A[i] = pk + off[i];
+    }
+    if ((rc = hip_status(launch_pack_a(c.kind, J, s), "pack A"))) return rc;
+  }
+  return hip_status(launch_chain_fused(c, s), "fused chain");
+}
+
 int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
              const long *K, const fg_chain_opts *opts, void *stream) {
   int rc;
@@ -327,43 +371,10 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
   int splits = 0;
   if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits))) return rc;
   if (fused) {
-    ChainArgs c{};
-    c.kind = kind;
-    c.num_gemm = num_gemm;
-    for (int i = 0; i < num_gemm; ++i) {
-      c.A[i] = A[i];
-      c.M[i] = M[i];
-      c.K[i] = K[i];
-    }
-    c.B = B;
-    c.C = C;
-    c.N = N;
-    c.tile_n = tn;
+    ChainArgs c = chain_args(kind, B, C, N, num_gemm, A, M, K, tn);
     ChainParams P;
     make_chain_params(c, &P);
-    if (chain_fused_aligned(P)) {
-      if (!o.a_packed) {
-        // pack every A_i into MFMA fragment order (per-stream scratch, one launch)
-        size_t off[CHAIN_MAXG], total = 0;
-        for (int i = 0; i < num_gemm; ++i) {
-          off[i] = total;
-          total += (packed_a_bytes(kind, M[i], K[i]) + 255) & ~(size_t)255;
-        }
-        char *pk = nullptr;
-        if ((rc = scratch(s, "apack", total, (void **)&pk))) return rc;
-        PackJob J{};
-        J.n = num_gemm;
-        for (int i = 0; i < num_gemm; ++i) {
-          J.A[i] = A[i];
-          J.M[i] = M[i];
-          J.K[i] = K[i];
-          J.out[i] = pk + off[i];
-          c.A[i] = pk + off[i];
-        }
-        if ((rc = hip_status(launch_pack_a(kind, J, s), "pack A"))) return rc;
-      }
-      return hip_status(launch_chain_fused(c, s), "fused chain");
-    }
+    if (chain_fused_aligned(P)) return run_fused(c, o.a_packed, s);
     if (o.fuse == FG_FUSE_ON || o.a_packed)
       return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but B is not aligned to 4 elements");
     // misaligned B: run unfused
@@ -410,6 +421,96 @@ int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, i
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
   int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
   return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------- convolution front end --
+
+// Checks a convolution geometry and fills the kernels' ImgGeom. The padded image
+// must hold fewer than 2^31 elements (every offset the kernels form is 32-bit), and
+// K1 and N must stay below 2^31.
+static int conv_geom(const fg_conv_geom *g, ImgGeom *ig, long *K1, long *N) {
+  constexpr long lim = 1L << 31;
+  if (!g) return set_error(FG_ERR_INVALID, "conv geometry is NULL");
+  if (g->batch < 1 || g->channels < 1 || g->height < 1 || g->width < 1 || g->kernel_h < 1 || g->kernel_w < 1 ||
+      g->stride_h < 1 || g->stride_w < 1 || g->pad_h < 0 || g->pad_w < 0 || g->dilation_h < 1 ||
+      g->dilation_w < 1 || g->batch >= lim || g->channels >= lim || g->height >= lim || g->width >= lim ||
+      g->kernel_h >= lim || g->kernel_w >= lim || g->stride_h >= lim || g->stride_w >= lim || g->pad_h >= lim ||
+      g->pad_w >= lim || g->dilation_h >= lim || g->dilation_w >= lim)
+    return set_error(FG_ERR_INVALID, "conv geometry: a size is out of range");
+  const long ph = g->height + 2 * g->pad_h, pw = g->width + 2 * g->pad_w;
+  const long eh = g->dilation_h * (g->kernel_h - 1) + 1, ew = g->dilation_w * (g->kernel_w - 1) + 1;
+  if (ph < eh || pw < ew)
+    return set_error(FG_ERR_INVALID, "conv geometry: the %ldx%ld kernel extent exceeds the padded %ldx%ld image", eh,
+                     ew, ph, pw);
+  const long oh = (ph - eh) / g->stride_h + 1, ow = (pw - ew) / g->stride_w + 1;
+  // products checked one factor at a time against 2^31
+  auto below = [](long a, long b) { return a < lim && b < lim && (b == 0 || a < lim / b); };
+  if (!below(ph, pw) || !below(ph * pw, g->channels) || !below(ph * pw * g->channels, g->batch))
+    return set_error(FG_ERR_INVALID, "conv geometry: the padded image has 2^31 elements or more");
+  if (!below(oh, ow) || !below(oh * ow, g->batch) || !below(g->kernel_h, g->kernel_w) ||
+      !below(g->kernel_h * g->kernel_w, g->channels))
+    return set_error(FG_ERR_INVALID, "conv geometry: K1 or N reaches 2^31");
+  if (ig) {
+    ig->C = (int)g->channels;
+    ig->H = (int)g->height;
+    ig->W = (int)g->width;
+    ig->kh = (int)g->kernel_h;
+    ig->kw = (int)g->kernel_w;
+    ig->sh = (int)g->stride_h;
+    ig->sw = (int)g->stride_w;
+    ig->ph = (int)g->pad_h;
+    ig->pw = (int)g->pad_w;
+    ig->dh = (int)g->dilation_h;
+    ig->dw = (int)g->dilation_w;
+    ig->OH = (int)oh;
+    ig->OW = (int)ow;
+  }
+  if (K1) *K1 = g->channels * g->kernel_h * g->kernel_w;
+  if (N) *N = g->batch * oh * ow;
+  return FG_OK;
+}
+
+int fg_conv_dims(const fg_conv_geom *g, long *K1, long *N) { return conv_geom(g, nullptr, K1, N); }
+
+int fg_im2col(int dtype, void *B, const void *X, const fg_conv_geom *g, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8) return set_error(FG_ERR_INVALID, "dtype %d", dtype);
+  if (!B || !X) return set_error(FG_ERR_INVALID, "im2col: B or X is NULL");
+  ImgGeom ig;
+  long K1, N;
+  if ((rc = conv_geom(g, &ig, &K1, &N))) return rc;
+  return hip_status(launch_im2col((int)dtype_bytes(dtype), X, B, ig, K1, N, (hipStream_t)stream), "im2col");
+}
+
+int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
+                  const long *M, const long *K, const fg_chain_opts *opts, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  ImgGeom ig;
+  long K1, N;
+  if ((rc = conv_geom(g, &ig, &K1, &N))) return rc;
+  if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
+  if (K[0] != K1) return set_error(FG_ERR_INVALID, "conv chain: K[0]=%ld but the patches have K1=%ld rows", K[0], K1);
+  if (!X || !C) return set_error(FG_ERR_INVALID, "conv chain: X or C is NULL");
+  fg_chain_opts o;
+  if (opts) o = *opts; else fg_chain_opts_default(&o);
+  hipStream_t s = (hipStream_t)stream;
+  const int kind = kind_of(dtype, o.f32_mode);
+  // The gathering kernel loads single elements, so the fused path takes any N: plan
+  // with N rounded up to whole 4-column groups.
+  int fused = 0, tn = 0, splits = 0;
+  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &fused, &tn, &splits))) return rc;
+  if (fused) {
+    ChainArgs c = chain_args(kind, X, C, N, num_gemm, A, M, K, tn);
+    c.img = &ig;
+    return run_fused(c, o.a_packed, s);
+  }
+  // unfused (options, LDS or the reference layout): form B, then the matrix chain
+  void *B;
+  if ((rc = scratch(s, "im2col", (size_t)K1 * N * dtype_bytes(dtype), &B))) return rc;
+  if ((rc = hip_status(launch_im2col((int)dtype_bytes(dtype), X, B, ig, K1, N, s), "im2col"))) return rc;
+  return fg_chain(dtype, C, B, N, num_gemm, A, M, K, &o, stream);
 }
 
 // -------------------------------------------------------- host-pointer forms --

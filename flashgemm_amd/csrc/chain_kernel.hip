@@ -205,10 +205,11 @@ __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KI
 // read a clamped address and are zeroed with a select. `issue` runs after the first
 // round of tile loads and before their LDS writes: the caller's weight prefetch goes
 // there, so the tile's loads are not queued behind it. INT8 adds each column's sum of
-// s8 X_0 into colsum (zeroed beforehand).
-template <int KIND, int TN, typename F>
+// s8 X_0 into colsum (zeroed beforehand). IMG: B is the patch matrix of the image at
+// B (geometry img), gathered element by element; N % 4 and alignment do not matter.
+template <int KIND, int TN, bool IMG, typename F>
 __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K1p, long N, long n0,
-                                        int *colsum, F &&issue) {
+                                        int *colsum, const ImgGeom &img, F &&issue) {
   using T = CT<KIND>;
   // one task = KG rows x 4 columns of one k-group
   const int ngroups = (int)(K1p / T::KG);
@@ -224,7 +225,73 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
   // tasks per thread loaded before `issue` (BF16 tiles of 64 columns have two)
   constexpr int FIRST = KIND == K_BF16 ? 2 : 1;
   Raw x[FIRST][T::KG];
+  // Patch gather: column n = (b*OH + oy)*OW + ox, row k = (c*kh + ky)*kw + kx reads
+  // pixel (oy*sh - ph + ky*dh, ox*sw - pw + kx*dw) of channel c of image b, zero
+  // outside the image. One division per task for its first column and first row;
+  // the other columns and rows step their coordinates. Lanes outside the matrix or
+  // the image load element 0 and select zero.
+  auto load_img = [&](int task, Raw(&x)[T::KG]) {
+    const int g = task / (TN / 4), cb = task % (TN / 4);
+    const long nb = n0 + cb * 4;
+    const int ohw = img.OH * img.OW, HW = img.H * img.W;
+    int cy[4], cx[4], cbase[4];
+    bool cok[4];
+    {
+      const int n = (int)(nb < N ? nb : 0);
+      int b = n / ohw, r = n - b * ohw;
+      int oy = r / img.OW, ox = r - oy * img.OW;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        cok[j] = nb + j < N;
+        cy[j] = oy * img.sh - img.ph;
+        cx[j] = ox * img.sw - img.pw;
+        cbase[j] = (b * img.C * img.H + cy[j]) * img.W + cx[j];
+        if (++ox == img.OW) {
+          ox = 0;
+          if (++oy == img.OH) {
+            oy = 0;
+            ++b;
+          }
+        }
+      }
+    }
+    const int k0 = g * T::KG, kk = img.kh * img.kw;
+    int c = k0 / kk, rr = k0 - c * kk;
+    int ky = rr / img.kw, kx = rr - ky * img.kw;
+#pragma unroll
+    for (int r = 0; r < T::KG; ++r) {
+      const bool rok = k0 + r < K1;
+      const int dy = ky * img.dh, dx = kx * img.dw;
+      const int roff = c * HW + dy * img.W + dx;
+      uint32_t e[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool ok = rok && cok[j] && (unsigned)(cy[j] + dy) < (unsigned)img.H &&
+                        (unsigned)(cx[j] + dx) < (unsigned)img.W;
+        const unsigned off = ok ? (unsigned)(cbase[j] + roff) : 0u;
+        uint32_t v;
+        if constexpr (T::ESZ == 4) v = *(const uint32_t *)(B + (size_t)off * 4);
+        else if constexpr (T::ESZ == 2) v = *(const uint16_t *)(B + (size_t)off * 2);
+        else v = *(const uint8_t *)(B + off);
+        e[j] = ok ? v : 0u;
+      }
+      if constexpr (T::ESZ == 4) x[r] = make_uint4(e[0], e[1], e[2], e[3]);
+      else if constexpr (T::ESZ == 2) x[r] = make_uint2(e[0] | (e[1] << 16), e[2] | (e[3] << 16));
+      else x[r] = e[0] | (e[1] << 8) | (e[2] << 16) | (e[3] << 24);
+      if (++kx == img.kw) {
+        kx = 0;
+        if (++ky == img.kh) {
+          ky = 0;
+          ++c;
+        }
+      }
+    }
+  };
   auto load = [&](int task, Raw(&x)[T::KG]) {
+    if constexpr (IMG) {
+      load_img(task, x);
+      return;
+    }
     const int g = task / (TN / 4), cb = task % (TN / 4);
     const long nb = n0 + cb * 4, k0 = (long)g * T::KG;
     const long nbc = nb < N ? nb : 0;
@@ -439,7 +506,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 // buffered a whole chunk ahead: while chunk c's MFMAs run, chunk c+1's weight
 // loads (global) and B reads (LDS) are in flight. The weight stream continues
 // across row passes and GEMM boundaries.
-template <int KIND, int TNB>
+template <int KIND, int TNB, bool IMG = false>
 __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
@@ -506,8 +573,8 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   const long K1c = (P.K[0] + CS - 1) / CS * CS;
   // A wave with no rows in GEMM 0 loads (and ignores) wave 0's first chunk, which
   // always exists: the loads stay unconditional and the wait counts after them exact.
-  stage_b<KIND, TN>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum,
-                    [&] { load_chunk(apk ? apk : (const char *)P.A[0], fa[0]); });
+  stage_b<KIND, TN, IMG>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img,
+                         [&] { load_chunk(apk ? apk : (const char *)P.A[0], fa[0]); });
   __syncthreads();
 
   // rows [Mp, Mc) of Y_gi (the k-padding of the next GEMM) must read as zero
@@ -651,10 +718,10 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <int KIND, int TNB>
+template <int KIND, int TNB, bool IMG>
 __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   extern __shared__ uint4 arena[];
-  chain_tile<KIND, TNB>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
+  chain_tile<KIND, TNB, IMG>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
 }
 
 constexpr int RP_of(int kind) {
@@ -718,11 +785,11 @@ template <int KIND, int TNB>
 hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   ChainParams P{};
   make_chain_params(c, &P);
-  if (!chain_fused_aligned(P)) return hipErrorInvalidValue;
+  if (!c.img && !chain_fused_aligned(P)) return hipErrorInvalidValue;
   constexpr int TN = TNB * 32;
   const size_t lds = arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
-  auto kfn = chain_kernel<KIND, TNB>;
+  auto kfn = c.img ? chain_kernel<KIND, TNB, true> : chain_kernel<KIND, TNB, false>;
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((unsigned)((c.N + TN - 1) / TN));
@@ -851,6 +918,7 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
   P->C = c.C;
   P->N = c.N;
   P->vecB = (c.N % 4 == 0) && (((uintptr_t)c.B & (4 * e - 1)) == 0);
+  if (c.img) P->img = *c.img;
 }
 
 bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K) {

@@ -158,15 +158,28 @@ hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, c
 
 constexpr int CHAIN_MAXG = 8;
 
+// Image geometry of a chain whose B is the patch matrix of an NCHW image (im2col):
+// B[k][n] = X[b][c][oy*sh - ph + ky*dh][ox*sw - pw + kx*dw] (0 outside the image),
+// k = (c*kh + ky)*kw + kx, n = (b*OH + oy)*OW + ox. The image holds fewer than 2^31
+// elements, so element offsets are 32-bit.
+struct ImgGeom {
+  int C, H, W;
+  int kh, kw, sh, sw, ph, pw, dh, dw;
+  int OH, OW;
+};
+// Element size `esz` (1, 2 or 4 bytes): B (K1 x N, K1 = C*kh*kw, N = batch*OH*OW).
+hipError_t launch_im2col(int esz, const void *X, void *B, const ImgGeom &g, long K1, long N, hipStream_t s);
+
 struct ChainArgs {
   int kind;                 // Kind
   int num_gemm;
   const void *A[CHAIN_MAXG];  // packed weights for the fused kernels
   long M[CHAIN_MAXG], K[CHAIN_MAXG];
-  const void *B;            // K1 x N
+  const void *B;            // K1 x N, or the image when img is set
   void *C;                  // M_n x N (fp32 or int32)
   long N;
   int tile_n;               // columns per workgroup
+  const ImgGeom *img;       // nullptr: B is a matrix; else B is gathered from the image
 };
 
 // Device-side description of one chain for the fused kernels (filled on the host
@@ -185,11 +198,12 @@ struct ChainParams {
   void *C;
   long N;
   int vecB;
+  ImgGeom img;              // geometry when B is gathered from an image (chain_kernel IMG)
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);
 // The fused kernels stage B in groups of 4 columns: N % 4 == 0 (shape) and a B
-// base aligned to 4 elements.
+// base aligned to 4 elements (a B gathered from an image has neither requirement).
 bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K);
 bool chain_fused_aligned(const ChainParams &P);
 

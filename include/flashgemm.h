@@ -127,6 +127,40 @@ int fg_pack_weight(int dtype, int f32_mode, const void *A, long M, long K, void 
 int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
                   int *fused, int *tile_n);
 
+/* ---- convolution front end: B produced from an image (im2col). The reference's
+ * chains are convolution layers written as GEMMs: its test shapes take K_1 =
+ * channels * 3 * 3 or * 5 * 5 and N = height * width (multigemm/test/test_f32_3.c:14-48),
+ * and its callers lay the patches out as B before the call. Here the fused chain
+ * gathers the patches straight from the image while it stages its B tile, so B
+ * never exists in HBM. ----
+ *
+ * X is batch x channels x height x width, row-major (NCHW), of the chain's dtype
+ * (fp32, raw bf16 or int8). B[k][n] = X[b][c][iy][ix] with
+ *   k = (c * kernel_h + ky) * kernel_w + kx,   n = (b * out_h + oy) * out_w + ox,
+ *   iy = oy * stride_h - pad_h + ky * dilation_h,  ix = ox * stride_w - pad_w + kx * dilation_w,
+ * and 0 where (iy, ix) falls outside the image (the row order of torch's unfold).
+ * out_h = (height + 2 pad_h - dilation_h (kernel_h - 1) - 1) / stride_h + 1, out_w alike;
+ * K_1 = channels * kernel_h * kernel_w, N = batch * out_h * out_w. */
+typedef struct fg_conv_geom {
+  long batch, channels, height, width;
+  long kernel_h, kernel_w;
+  long stride_h, stride_w;     /* >= 1 */
+  long pad_h, pad_w;           /* >= 0 */
+  long dilation_h, dilation_w; /* >= 1 */
+} fg_conv_geom;
+
+/* Output size of the patch matrix: *K1 = channels * kernel_h * kernel_w, *N =
+ * batch * out_h * out_w (either pointer may be NULL). FG_ERR_INVALID for a bad
+ * geometry (non-positive sizes, empty output, an image of 2^31 elements or more). */
+int fg_conv_dims(const fg_conv_geom *g, long *K1, long *N);
+/* B = im2col(X) on device pointers: B is K1 x N row-major of the dtype. */
+int fg_im2col(int dtype, void *B, const void *X, const fg_conv_geom *g, void *stream);
+/* C = A_n ... A_1 im2col(X) on device pointers; K[0] must equal K1 of g and C is
+ * M_n x N. The fused chain gathers its B tile from X; when the chain runs unfused
+ * (options, LDS, the reference layout) B is formed in library scratch first. */
+int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
+                  const long *M, const long *K, const fg_chain_opts *opts, void *stream);
+
 /* ---- grouped launches (no reference counterpart: the reference runs one chain
  * per call). Many chains of uneven shapes and mixed dtypes, mathematical layout,
  * all pointers device pointers. ---- */

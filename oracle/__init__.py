@@ -157,3 +157,31 @@ def rel_fro(x: np.ndarray, ref: np.ndarray) -> float:
     ref = np.asarray(ref, np.float64)
     den = np.linalg.norm(ref)
     return float(np.linalg.norm(x - ref) / (den if den > 0 else 1.0))
+
+
+def im2col(X: np.ndarray, kernel_h: int, kernel_w: int, stride=(1, 1), pad=(0, 0), dilation=(1, 1)) -> np.ndarray:
+    """Patch matrix of an NCHW image: B[k][n] = X[b][c][iy][ix], zero outside the image.
+
+    k = (c * kernel_h + ky) * kernel_w + kx, n = (b * OH + oy) * OW + ox,
+    iy = oy * stride_h - pad_h + ky * dilation_h, ix = ox * stride_w - pad_w + kx * dilation_w.
+    This front end has no reference counterpart: the reference's callers form B
+    themselves (its conv-shaped tests, multigemm/test/test_f32_3.c:14-48, pass a
+    random B). The row order is torch's unfold order; tests pin it against
+    torch.nn.functional.unfold and a direct convolution. Any dtype (a pure gather).
+    """
+    X = np.asarray(X)
+    nb, C, H, W = X.shape
+    sh, sw = stride
+    ph, pw = pad
+    dh, dw = dilation
+    OH = (H + 2 * ph - dh * (kernel_h - 1) - 1) // sh + 1
+    OW = (W + 2 * pw - dw * (kernel_w - 1) - 1) // sw + 1
+    Xp = np.zeros((nb, C, H + 2 * ph, W + 2 * pw), X.dtype)
+    Xp[:, :, ph:ph + H, pw:pw + W] = X
+    B = np.empty((C, kernel_h, kernel_w, nb, OH, OW), X.dtype)
+    for ky in range(kernel_h):
+        for kx in range(kernel_w):
+            y0, x0 = ky * dh, kx * dw
+            win = Xp[:, :, y0:y0 + sh * (OH - 1) + 1:sh, x0:x0 + sw * (OW - 1) + 1:sw]  # nb, C, OH, OW
+            B[:, ky, kx] = win.transpose(1, 0, 2, 3)
+    return B.reshape(C * kernel_h * kernel_w, nb * OH * OW)

@@ -164,6 +164,18 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
         torch.cuda.synchronize()
         if not torch.equal(Ck, C):
             raise RuntimeError("pre-packed chain differs from the drop-in call")
+    # the same chain with every epilogue extension on (per-channel bias and scale,
+    # INT8 zero point, ReLU): the cost of the extended epilogue in the fused kernel
+    if dtype == fg.DTYPE_INT8:
+        bias = [torch.randint(-2 ** 15, 2 ** 15, (m,), generator=gen, device="cuda", dtype=torch.int32) for m in Ms]
+        scale = [torch.full((m,), 1.0 / (k * 64), device="cuda") for m, k in zip(Ms[:-1], Ks[:-1])] + [None]
+        epi = fg.epilogue(bias=bias, scale=scale, zero_point=[5] * (len(Ms) - 1), relu=[1] * len(Ms))
+    else:
+        bias = [torch.zeros((m,), device="cuda") for m in Ms]
+        scale = [torch.ones((m,), device="cuda") for m in Ms]
+        epi = fg.epilogue(bias=bias, scale=scale, relu=[1] * len(Ms))
+    Ce = torch.empty_like(C)
+    ms_epi = timer.run(lambda: fg.chain_epi(dtype, Ce, B, As, epi, stream=stream), steps, warmup)
     Cu = torch.empty_like(C)
     ms_unfused = timer.run(lambda: fg.chain(dtype, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF), stream=stream),
                            steps, warmup)
@@ -173,7 +185,8 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
     flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
     peak = PEAK["int8" if dtype == fg.DTYPE_INT8 else "bf16"]
     out = dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / peak, fused=fused, tile_n=tn,
-               ms_one_launch_per_gemm=ms_unfused, fused_vs_unfused=(same if dtype == fg.DTYPE_INT8 else None),
+               ms_one_launch_per_gemm=ms_unfused, ms_with_epilogue_extensions=ms_epi,
+               fused_vs_unfused=(same if dtype == fg.DTYPE_INT8 else None),
                fused_vs_unfused_rel=(None if dtype == fg.DTYPE_INT8 else same))
     if ms_kernel is not None:
         out.update(ms_kernel=ms_kernel, kernel_tflops=flop / ms_kernel / 1e9,

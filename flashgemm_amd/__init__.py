@@ -68,6 +68,27 @@ def conv_geom(batch: int, channels: int, height: int, width: int, kernel_h: int,
                     dilation if dilation_w is None else dilation_w)
 
 
+class Epilogue(ctypes.Structure):
+    """fg_epilogue: per-GEMM bias / scale (device pointers), INT8 zero point, ReLU."""
+    _fields_ = [("bias", ctypes.c_void_p * 8), ("scale", ctypes.c_void_p * 8), ("zero_point", ctypes.c_int * 8),
+                ("relu", ctypes.c_int * 8)]
+
+
+def epilogue(bias=None, scale=None, zero_point=None, relu=None) -> Epilogue:
+    """Build an fg_epilogue from per-GEMM lists (None entries: not set). bias and scale
+    entries are device buffers (torch tensors or addresses)."""
+    e = Epilogue()
+    for i, b in enumerate(bias or []):
+        e.bias[i] = _addr(b) if b is not None else None
+    for i, sc in enumerate(scale or []):
+        e.scale[i] = _addr(sc) if sc is not None else None
+    for i, z in enumerate(zero_point or []):
+        e.zero_point[i] = int(z)
+    for i, r in enumerate(relu or []):
+        e.relu[i] = int(bool(r))
+    return e
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
                 ("num_streams", ctypes.c_int), ("max_batch", ctypes.c_int), ("use_graph", ctypes.c_int),
@@ -93,7 +114,9 @@ def lib() -> ctypes.CDLL:
         L.fg_conv_dims.argtypes = [ctypes.POINTER(ConvGeom), lp, lp]
         L.fg_im2col.argtypes = [i, vp, vp, ctypes.POINTER(ConvGeom), vp]
         L.fg_chain_conv.argtypes = [i, vp, vp, ctypes.POINTER(ConvGeom), i, vpp, lp, lp,
-                                    ctypes.POINTER(ChainOpts), vp]
+                                    ctypes.POINTER(Epilogue), ctypes.POINTER(ChainOpts), vp]
+        L.fg_chain_epi.argtypes = [i, vp, vp, l, i, vpp, lp, lp, ctypes.POINTER(Epilogue),
+                                   ctypes.POINTER(ChainOpts), vp]
         L.fg_sgemm.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_host.argtypes = [vp, vp, vp, l, l, l, i]
         L.fg_f64_chain.argtypes = [vp, vp, l, i, vpp, lp, lp, vp]
@@ -219,11 +242,22 @@ def im2col(dtype: int, B, X, g: ConvGeom, stream=None) -> None:
 
 
 def chain_conv(dtype: int, C, X, g: ConvGeom, As, Ms=None, Ks=None, o: ChainOpts | None = None,
-               stream=None) -> None:
+               stream=None, epi: Epilogue | None = None) -> None:
     """C = A_n ... A_1 im2col(X) on device buffers; the fused chain gathers its B tile from X."""
     n, ptrs, M, K = _arrays(As, Ms, Ks)
     _check(lib().fg_chain_conv(dtype, _addr(C), _addr(X), ctypes.byref(g), n, ptrs, M, K,
+                               ctypes.byref(epi) if epi is not None else None,
                                ctypes.byref(o) if o is not None else None, _stream(stream)))
+
+
+def chain_epi(dtype: int, C, B, As, epi: Epilogue | None, N: int | None = None, Ms=None, Ks=None,
+              o: ChainOpts | None = None, stream=None) -> None:
+    """fg_chain with epilogue extensions (per-GEMM bias, scale, INT8 zero point, ReLU)."""
+    n, ptrs, M, K = _arrays(As, Ms, Ks)
+    N = N if N is not None else B.shape[1]
+    _check(lib().fg_chain_epi(dtype, _addr(C), _addr(B), N, n, ptrs, M, K,
+                              ctypes.byref(epi) if epi is not None else None,
+                              ctypes.byref(o) if o is not None else None, _stream(stream)))
 
 
 def sgemm(C, A, B, M: int, N: int, K: int, f32_mode: int = F32_MFMA, splits: int = 0, stream=None) -> None:

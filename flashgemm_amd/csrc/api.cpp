@@ -272,9 +272,11 @@ int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K,
   return plan_chain(dtype, N, num_gemm, M, K, opts, fused, tile_n, &splits);
 }
 
-// One GEMM (+ INT8 column sums, + split-K reduction) of the unfused path.
+// One GEMM (+ INT8 column sums, + split-K reduction) of the unfused path. With
+// epilogue extensions (ex != nullptr) the GEMM always writes fp32 / int32 slabs and
+// the reduction pass applies the extensions while it forms the output.
 static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K, int splits,
-                    hipStream_t s) {
+                    hipStream_t s, const EpiOne *ex = nullptr) {
   int *colsum = nullptr;
   int rc;
   if (kind == K_F32 && epi != EPI_PARTIAL && splits <= 0 && skinny_f32_shape(M, N)) {
@@ -282,15 +284,15 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
     long kper = 0;
     const int sk = skinny_f32_splits(M, N, K, &kper);
     float *out = (float *)C;
-    if (sk > 1 && (rc = scratch(s, "partials", (size_t)sk * M * N * 4, (void **)&out))) return rc;
+    if ((sk > 1 || ex) && (rc = scratch(s, "partials", (size_t)sk * M * N * 4, (void **)&out))) return rc;
     if ((rc = hip_status(launch_skinny_f32((const float *)A, (const float *)B, out, M, N, K, kper, sk, s),
                          "skinny gemm")))
       return rc;
-    if (sk == 1) return FG_OK;
-    return hip_status(launch_splitk_reduce(kind, EPI_STORE, out, sk, M, N, nullptr, C, s), "split-K reduce");
+    if (sk == 1 && !ex) return FG_OK;
+    return hip_status(launch_splitk_reduce(kind, EPI_STORE, out, sk, M, N, nullptr, C, ex, s), "split-K reduce");
   }
   if (splits <= 0) splits = plan_splits(M, N, K, kind);
-  if (splits <= 1) {  // INT8: the kernel sums its own B columns (colsum == nullptr)
+  if (splits <= 1 && !ex) {  // INT8: the kernel sums its own B columns (colsum == nullptr)
     GemmArgs g{kind, epi, A, B, C, nullptr, M, N, K, 1};
     return hip_status(launch_gemm(g, s), "gemm");
   }
@@ -306,7 +308,7 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
   long bk = (kind == K_I8) ? 128 : (kind == K_BF16 ? 64 : 32);
   long kper = ((K + splits - 1) / splits + bk - 1) / bk * bk;
   int used = (int)((K + kper - 1) / kper);
-  return hip_status(launch_splitk_reduce(kind, epi, part, used, M, N, colsum, C, s), "split-K reduce");
+  return hip_status(launch_splitk_reduce(kind, epi, part, used, M, N, colsum, C, ex, s), "split-K reduce");
 }
 
 // Arguments of a fused launch (A[i] as given; run_fused packs them unless a_packed).
@@ -353,8 +355,48 @@ static int run_fused(ChainArgs c, int a_packed, hipStream_t s) {
   return hip_status(launch_chain_fused(c, s), "fused chain");
 }
 
+// Epilogue extensions from the public struct; *out = nullptr when every field is at
+// its default (the reference's epilogues, the plain kernels).
+static int epi_params(int dtype, int num_gemm, const fg_epilogue *e, EpiParams *p, const EpiParams **out) {
+  *out = nullptr;
+  if (!e) return FG_OK;
+  bool any = false;
+  for (int i = 0; i < num_gemm; ++i) {
+    const bool last = i == num_gemm - 1;
+    if (dtype == FG_DTYPE_INT8 && !last && (e->zero_point[i] < -128 || e->zero_point[i] > 127))
+      return set_error(FG_ERR_INVALID, "epilogue: zero_point[%d] = %d (-128..127)", i, e->zero_point[i]);
+    if ((dtype != FG_DTYPE_INT8 || last) && e->zero_point[i] != 0)
+      return set_error(FG_ERR_UNSUPPORTED, "epilogue: zero_point[%d] applies to INT8 intermediates only", i);
+    if (dtype == FG_DTYPE_INT8 && last && e->scale[i])
+      return set_error(FG_ERR_UNSUPPORTED, "epilogue: the INT8 chain's int32 output takes no scale");
+    p->g[i] = EpiOne{e->bias[i], e->scale[i], e->zero_point[i], e->relu[i] != 0};
+    any = any || e->bias[i] || e->scale[i] || e->zero_point[i] || e->relu[i];
+  }
+  if (any) *out = p;
+  return FG_OK;
+}
+
+static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+                      const long *K, const EpiParams *ep, const fg_chain_opts *opts, void *stream);
+
 int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
              const long *K, const fg_chain_opts *opts, void *stream) {
+  return chain_impl(dtype, C, B, N, num_gemm, A, M, K, nullptr, opts, stream);
+}
+
+int fg_chain_epi(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+                 const long *K, const fg_epilogue *epi, const fg_chain_opts *opts, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
+  EpiParams p{};
+  const EpiParams *ep;
+  if ((rc = epi_params(dtype, num_gemm, epi, &p, &ep))) return rc;
+  return chain_impl(dtype, C, B, N, num_gemm, A, M, K, ep, opts, stream);
+}
+
+static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+                      const long *K, const EpiParams *ep, const fg_chain_opts *opts, void *stream) {
   int rc;
   if ((rc = ensure_init())) return rc;
   if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
@@ -372,6 +414,7 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
   if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits))) return rc;
   if (fused) {
     ChainArgs c = chain_args(kind, B, C, N, num_gemm, A, M, K, tn);
+    c.epi = ep;
     ChainParams P;
     make_chain_params(c, &P);
     if (chain_fused_aligned(P)) return run_fused(c, o.a_packed, s);
@@ -398,7 +441,9 @@ int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void
   for (int i = 0; i < num_gemm; ++i) {
     bool last = i == num_gemm - 1;
     void *Y = last ? C : buf[i & 1];
-    if ((rc = run_gemm(kind, last ? EPI_STORE : EPI_REQUANT, A[i], X, Y, M[i], N, K[i], splits, s))) return rc;
+    if ((rc = run_gemm(kind, last ? EPI_STORE : EPI_REQUANT, A[i], X, Y, M[i], N, K[i], splits, s,
+                       ep ? &ep->g[i] : nullptr)))
+      return rc;
     if (!last && ref) {
       if ((rc = hip_status(launch_ref_permute(kind == K_I8, Y, buf[2], M[i], panels, s), "ref permute")))
         return rc;
@@ -484,7 +529,7 @@ int fg_im2col(int dtype, void *B, const void *X, const fg_conv_geom *g, void *st
 }
 
 int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
-                  const long *M, const long *K, const fg_chain_opts *opts, void *stream) {
+                  const long *M, const long *K, const fg_epilogue *epi, const fg_chain_opts *opts, void *stream) {
   int rc;
   if ((rc = ensure_init())) return rc;
   ImgGeom ig;
@@ -493,6 +538,9 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
   if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
   if (K[0] != K1) return set_error(FG_ERR_INVALID, "conv chain: K[0]=%ld but the patches have K1=%ld rows", K[0], K1);
   if (!X || !C) return set_error(FG_ERR_INVALID, "conv chain: X or C is NULL");
+  EpiParams p{};
+  const EpiParams *ep;
+  if ((rc = epi_params(dtype, num_gemm, epi, &p, &ep))) return rc;
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
   hipStream_t s = (hipStream_t)stream;
@@ -504,13 +552,14 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
   if (fused) {
     ChainArgs c = chain_args(kind, X, C, N, num_gemm, A, M, K, tn);
     c.img = &ig;
+    c.epi = ep;
     return run_fused(c, o.a_packed, s);
   }
   // unfused (options, LDS or the reference layout): form B, then the matrix chain
   void *B;
   if ((rc = scratch(s, "im2col", (size_t)K1 * N * dtype_bytes(dtype), &B))) return rc;
   if ((rc = hip_status(launch_im2col((int)dtype_bytes(dtype), X, B, ig, K1, N, s), "im2col"))) return rc;
-  return fg_chain(dtype, C, B, N, num_gemm, A, M, K, &o, stream);
+  return chain_impl(dtype, C, B, N, num_gemm, A, M, K, ep, &o, stream);
 }
 
 // -------------------------------------------------------- host-pointer forms --

@@ -397,8 +397,10 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
 // row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena (INT8
 // also adds the s8 column sums of what it writes into csum: the next GEMM's bias).
 // Index math is 32-bit (arena rows and M fit easily) and the row-bound checks are
-// only evaluated in the block that straddles M.
-template <int KIND, int TNB>
+// only evaluated in the block that straddles M. EPX applies GEMM gi's epilogue
+// extensions (P.epi.g[gi]: per-row bias and scale, zero point, ReLU; epi_i8, epi_i32,
+// epi_f); rows past M read no bias or scale.
+template <int KIND, int TNB, bool EPX>
 __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, long rb0_l, long n0, uint4 *arena,
                                                 const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB], int *csum) {
   using T = CT<KIND>;
@@ -411,6 +413,9 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   const int h = lane >> 5, l32 = lane & 31;
   const int M = (int)P.M[gi], Mp = (M + 31) & ~31;
   const int rb0 = (int)rb0_l;
+  const EpiOne ex = EPX ? P.epi.g[gi] : EpiOne{};
+  const int *const bias_i = (const int *)ex.bias;
+  const float *const bias_f = (const float *)ex.bias;
   if (gi == P.num_gemm - 1) {
     const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
     const size_t rs = (size_t)P.N * 4;  // bytes per row of C
@@ -427,9 +432,19 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int ro = 8 * (r >> 2) + (r & 3);
-          if (full || rbase + 4 * h + ro < M) {
-            if (KIND == K_I8) *(int *)(base + ro * rs) = (int)acc[i][j][r];
-            else *(float *)(base + ro * rs) = (float)acc[i][j][r];
+          const int m = rbase + 4 * h + ro;
+          if (full || m < M) {
+            if (KIND == K_I8) {
+              int v = (int)acc[i][j][r];
+              if (EPX) v = epi_i32(v, bias_i ? bias_i[m] : 0, ex.relu);
+              *(int *)(base + ro * rs) = v;
+            } else {
+              float v = (float)acc[i][j][r];
+              if (EPX)
+                v = epi_f(v, ex.scale != nullptr, ex.scale ? ex.scale[m] : 1.0f, bias_f != nullptr,
+                          bias_f ? bias_f[m] : 0.0f, ex.relu);
+              *(float *)(base + ro * rs) = v;
+            }
           }
         }
       }
@@ -459,19 +474,40 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
     for (int q = 0; q < 4; ++q) {
       const int m0r = rbase + 8 * q + 4 * h;
       const int valid = full ? 4 : (M - m0r < 0 ? 0 : (M - m0r > 4 ? 4 : M - m0r));
+      // per-row extension operands of the 4 rows (rows past M use row M - 1)
+      int eb[4];
+      float es[4], ef[4];
+      if (EPX) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int m = m0r + e < M ? m0r + e : M - 1;
+          eb[e] = (KIND == K_I8 && bias_i) ? bias_i[m] : 0;
+          ef[e] = (KIND != K_I8 && bias_f) ? bias_f[m] : 0.0f;
+          es[e] = ex.scale ? ex.scale[m] : 1.0f;
+        }
+      }
 #pragma unroll
       for (int j = 0; j < TNB; ++j) {
         char *const dst = blk + ((8 * q / T::KG) * TN + j * 32) * 16 + (8 * q % T::KG) * EB;
         if (KIND == K_I8) {
-          uint32_t w = sat_pk_i8((int)acc[i][j][4 * q], (int)acc[i][j][4 * q + 1], (int)acc[i][j][4 * q + 2],
-                                 (int)acc[i][j][4 * q + 3]);
+          int a4[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            a4[e] = (int)acc[i][j][4 * q + e];
+            if (EPX) a4[e] = epi_i8(a4[e], eb[e], ex.scale != nullptr, es[e], ex.zp, ex.relu);
+          }
+          uint32_t w = sat_pk_i8(a4[0], a4[1], a4[2], a4[3]);
           if (!full) w &= valid == 0 ? 0u : (0xffffffffu >> (8 * (4 - valid)));
           *(uint32_t *)dst = w;
           cs[j] = __builtin_amdgcn_sdot4((int)w, 0x01010101, cs[j], false);
         } else {
           float v[4];
 #pragma unroll
-          for (int e = 0; e < 4; ++e) v[e] = (full || e < valid) ? (float)acc[i][j][4 * q + e] : 0.0f;
+          for (int e = 0; e < 4; ++e) {
+            float a = (float)acc[i][j][4 * q + e];
+            if (EPX) a = epi_f(a, ex.scale != nullptr, es[e], bias_f != nullptr, ef[e], ex.relu);
+            v[e] = (full || e < valid) ? a : 0.0f;
+          }
           if (KIND == K_F32) {
             *(uint4 *)dst = make_uint4(__builtin_bit_cast(uint32_t, v[0]), __builtin_bit_cast(uint32_t, v[1]),
                                        __builtin_bit_cast(uint32_t, v[2]), __builtin_bit_cast(uint32_t, v[3]));
@@ -506,7 +542,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 // buffered a whole chunk ahead: while chunk c's MFMAs run, chunk c+1's weight
 // loads (global) and B reads (LDS) are in flight. The weight stream continues
 // across row passes and GEMM boundaries.
-template <int KIND, int TNB, bool IMG = false>
+template <int KIND, int TNB, bool IMG = false, bool EPX = false>
 __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
@@ -688,7 +724,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       }
       apk = after;
       if (final_pass && !last && P.hold) held_rb0 = rb0;
-      else store_rows_impl<KIND, TNB>(P, gi, rb0, n0, arena, acc, csum_next);
+      else store_rows_impl<KIND, TNB, EPX>(P, gi, rb0, n0, arena, acc, csum_next);
     }
     if ((long)wave * RP * 32 >= Mp && next_gemm) {  // no rows here: fetch the next GEMM's first chunk
       apk = next_gemm;
@@ -703,7 +739,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       continue;
     }
     __syncthreads();  // every wave is done reading X_gi
-    if (held_rb0 >= 0) store_rows_impl<KIND, TNB>(P, gi, held_rb0, n0, arena, acc, csum_next);
+    if (held_rb0 >= 0) store_rows_impl<KIND, TNB, EPX>(P, gi, held_rb0, n0, arena, acc, csum_next);
     zero_pad(gi);
     __syncthreads();  // Y_gi complete
   }
@@ -718,10 +754,10 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <int KIND, int TNB, bool IMG>
+template <int KIND, int TNB, bool IMG, bool EPX>
 __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   extern __shared__ uint4 arena[];
-  chain_tile<KIND, TNB, IMG>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
+  chain_tile<KIND, TNB, IMG, EPX>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
 }
 
 constexpr int RP_of(int kind) {
@@ -789,7 +825,8 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   constexpr int TN = TNB * 32;
   const size_t lds = arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
-  auto kfn = c.img ? chain_kernel<KIND, TNB, true> : chain_kernel<KIND, TNB, false>;
+  auto kfn = c.img ? (c.epi ? chain_kernel<KIND, TNB, true, true> : chain_kernel<KIND, TNB, true, false>)
+                   : (c.epi ? chain_kernel<KIND, TNB, false, true> : chain_kernel<KIND, TNB, false, false>);
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((unsigned)((c.N + TN - 1) / TN));
@@ -919,6 +956,7 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
   P->N = c.N;
   P->vecB = (c.N % 4 == 0) && (((uintptr_t)c.B & (4 * e - 1)) == 0);
   if (c.img) P->img = *c.img;
+  if (c.epi) P->epi = *c.epi;
 }
 
 bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K) {

@@ -128,6 +128,53 @@ __host__ __device__ inline bool ref_src(bool is_int8, long w, long k, long c, lo
   return false;
 }
 
+// ------------------------------------------------------------- epilogues --
+constexpr int CHAIN_MAXG = 8;
+
+// Epilogue extensions of one GEMM (row m = output channel). With every field at its
+// default (null pointers, zero point 0, no ReLU) the epilogue is the reference's:
+// int8 saturate, bf16 RNE, fp32 as is.
+struct EpiOne {
+  const void *bias;    // M entries: int32 (INT8) or fp32; nullptr: none
+  const float *scale;  // M fp32 entries; nullptr: none (ignored for the INT8 C output)
+  int zp;              // INT8 intermediates: zero point, -128..127
+  int relu;            // nonzero: clamp below at zero (INT8 intermediates: at zp)
+};
+struct EpiParams {
+  EpiOne g[CHAIN_MAXG];
+};
+
+// INT8 intermediate: a = acc + bias (wrapping int32); with a scale, t = fp32(a) *
+// scale (RNE conversion and product) rounded half-to-even to an integer, else t = a;
+// q = sat8(t + zp), then max(q, zp) under ReLU. t is clamped to [-512, 512] first,
+// which cannot change q because zp lies in [-128, 127].
+__device__ __forceinline__ int epi_i8(int acc, int bias, bool has_scale, float scale, int zp, int relu) {
+  const int a = (int)((unsigned)acc + (unsigned)bias);
+  int t;
+  if (has_scale) {
+    const float f = __fmul_rn((float)a, scale);
+    t = (int)__builtin_rintf(__builtin_fminf(__builtin_fmaxf(f, -512.0f), 512.0f));
+  } else {
+    t = a < -512 ? -512 : (a > 512 ? 512 : a);
+  }
+  int q = t + zp;
+  q = q < -128 ? -128 : (q > 127 ? 127 : q);
+  return relu && q < zp ? zp : q;
+}
+// INT8 C output (int32): acc + bias (wrapping), then max(v, 0) under ReLU.
+__device__ __forceinline__ int epi_i32(int acc, int bias, int relu) {
+  const int v = (int)((unsigned)acc + (unsigned)bias);
+  return relu && v < 0 ? 0 : v;
+}
+// FP32 / BF16: y = acc * scale, then + bias (two fp32 roundings, never fused), then
+// max(y, 0) under ReLU (a NaN becomes 0).
+__device__ __forceinline__ float epi_f(float acc, bool has_scale, float scale, bool has_bias, float bias, int relu) {
+  float y = acc;
+  if (has_scale) y = __fmul_rn(y, scale);
+  if (has_bias) y = __fadd_rn(y, bias);
+  return relu ? (y > 0.0f ? y : 0.0f) : y;
+}
+
 // ------------------------------------------------------------ launchers ---
 // Launch helpers implemented in the .hip translation units. All return a
 // hipError_t and never synchronize.
@@ -151,12 +198,13 @@ int skinny_f32_splits(long M, long N, long K, long *kper);
 hipError_t launch_skinny_f32(const float *A, const float *B, float *C, long M, long N, long K, long kper,
                              int splits, hipStream_t s);
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s);
+// Sums the slabs and applies the epilogue `epi`, with the extensions of `ex` when it
+// is not null (launch_splitk_reduce is also the epilogue pass of unfused GEMMs that
+// have extensions).
 hipError_t launch_splitk_reduce(int kind, int epi, const void *partials, int splits, long M, long N,
-                                const int *colsum, void *C, hipStream_t s);
+                                const int *colsum, void *C, const EpiOne *ex, hipStream_t s);
 hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, const RefPanels &p,
                               hipStream_t s);
-
-constexpr int CHAIN_MAXG = 8;
 
 // Image geometry of a chain whose B is the patch matrix of an NCHW image (im2col):
 // B[k][n] = X[b][c][oy*sh - ph + ky*dh][ox*sw - pw + kx*dw] (0 outside the image),
@@ -180,6 +228,7 @@ struct ChainArgs {
   long N;
   int tile_n;               // columns per workgroup
   const ImgGeom *img;       // nullptr: B is a matrix; else B is gathered from the image
+  const EpiParams *epi;     // nullptr: the reference's epilogues; else per-GEMM extensions
 };
 
 // Device-side description of one chain for the fused kernels (filled on the host
@@ -199,6 +248,7 @@ struct ChainParams {
   long N;
   int vecB;
   ImgGeom img;              // geometry when B is gathered from an image (chain_kernel IMG)
+  EpiParams epi;            // epilogue extensions (chain_kernel EPX)
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);

@@ -510,12 +510,13 @@ __global__ __launch_bounds__(256) void colsum_i8_kernel(const int8_t *__restrict
 
 // Split-K reduction: a workgroup owns RE consecutive elements; RL lanes per element
 // each sum the slabs z = l, l + RL, ... in order, and lane sums are combined in
-// lane order through LDS, so the result does not depend on scheduling.
+// lane order through LDS, so the result does not depend on scheduling. With
+// has_ex it also applies the epilogue extensions (epi_i8 / epi_f, row m = e / N).
 constexpr int RE = 64;
 template <int KIND, int EPI, int RL>
 __global__ __launch_bounds__(RE * RL) void splitk_reduce_kernel(const void *__restrict__ P, int splits, long M,
                                                                  long N, const int *__restrict__ colsum,
-                                                                 void *__restrict__ C) {
+                                                                 void *__restrict__ C, EpiOne ex, int has_ex) {
   using Sum = typename std::conditional<KIND == K_I8, unsigned, float>::type;
   __shared__ Sum part[RL][RE];
   const int x = threadIdx.x % RE, l = threadIdx.x / RE;
@@ -533,11 +534,23 @@ __global__ __launch_bounds__(RE * RL) void splitk_reduce_kernel(const void *__re
   if (l != 0 || e >= MN) return;
   Sum s = part[0][x];
   for (int q = 1; q < RL; ++q) s += part[q][x];
+  const long m = (long)(e / N);
   if (KIND == K_I8) {
     s += 128u * (unsigned)colsum[e % N];
-    if (EPI == EPI_STORE) ((int *)C)[e] = (int)s;
-    else ((int8_t *)C)[e] = sat_i8((int)s);
+    if (has_ex) {
+      const int bias = ex.bias ? ((const int *)ex.bias)[m] : 0;
+      if (EPI == EPI_STORE) ((int *)C)[e] = epi_i32((int)s, bias, ex.relu);
+      else ((int8_t *)C)[e] = (int8_t)epi_i8((int)s, bias, ex.scale != nullptr, ex.scale ? ex.scale[m] : 1.0f, ex.zp,
+                                            ex.relu);
+    } else if (EPI == EPI_STORE) {
+      ((int *)C)[e] = (int)s;
+    } else {
+      ((int8_t *)C)[e] = sat_i8((int)s);
+    }
   } else {
+    if (has_ex)
+      s = epi_f(s, ex.scale != nullptr, ex.scale ? ex.scale[m] : 1.0f, ex.bias != nullptr,
+                ex.bias ? ((const float *)ex.bias)[m] : 0.0f, ex.relu);
     if (KIND == K_BF16 && EPI == EPI_REQUANT) ((uint16_t *)C)[e] = bf16_dev(s);
     else ((float *)C)[e] = s;
   }
@@ -616,13 +629,16 @@ hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream
 }
 
 hipError_t launch_splitk_reduce(int kind, int epi, const void *P, int splits, long M, long N,
-                                const int *colsum, void *C, hipStream_t s) {
+                                const int *colsum, void *C, const EpiOne *ex, hipStream_t s) {
+  const EpiOne exv = ex ? *ex : EpiOne{};
+  const int has_ex = ex != nullptr;
   size_t MN = (size_t)M * N;
   dim3 grid((unsigned)((MN + RE - 1) / RE));
   // lanes per element: enough to keep every slab read in flight, at most 16
   const int rl = splits >= 16 ? 16 : splits >= 8 ? 8 : splits >= 4 ? 4 : splits >= 2 ? 2 : 1;
 #define FG_RED1(K_, E_, L_) \
-  hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_, L_>), grid, dim3(RE * L_), 0, s, P, splits, M, N, colsum, C)
+  hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_, L_>), grid, dim3(RE * L_), 0, s, P, splits, M, N, colsum, C, \
+                     exv, has_ex)
 #define FG_RED(K_, E_)                          \
   do {                                          \
     switch (rl) {                               \

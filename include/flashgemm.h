@@ -103,6 +103,30 @@ int fg_synchronize(void *stream);
 int fg_chain(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
              const long *K, const fg_chain_opts *opts, void *stream);
 
+/* ---- epilogue extensions (no reference counterpart: the reference only
+ * saturates its INT8 intermediate to int8, multigemm/src/kernel_int8.h:329-385, and
+ * rounds its BF16 intermediate to bf16, kernel_bf16.h:356-390). Per GEMM i, row m
+ * (output channel m of GEMM i):
+ *   INT8 intermediate: a = acc + bias[m] (int32, wrapping); t = rne(fp32(a) * scale[m])
+ *     (half-to-even) when scale is set, else t = a; q = sat8(t + zero_point);
+ *     ReLU: q = max(q, zero_point). The next GEMM reads q.
+ *   INT8 output (int32 C): c = acc + bias[m]; ReLU: max(c, 0). No scale, no zero point.
+ *   FP32 / BF16: y = acc * scale[m], then y + bias[m] (fp32, two roundings); ReLU:
+ *     max(y, 0) (NaN -> 0); a BF16 intermediate is then rounded to bf16.
+ * Every field at zero / NULL gives the reference's epilogue. bias, scale: device
+ * pointers to M_i entries (bias int32 for INT8, fp32 otherwise). ---- */
+typedef struct fg_epilogue {
+  const void *bias[8];
+  const float *scale[8];
+  int zero_point[8]; /* INT8 intermediates only, -128..127 */
+  int relu[8];
+} fg_epilogue;
+
+/* fg_chain with epilogue extensions (epi may be NULL). FG_ERR_UNSUPPORTED for a
+ * zero point outside INT8 intermediates or a scale on the INT8 int32 output. */
+int fg_chain_epi(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
+                 const long *K, const fg_epilogue *epi, const fg_chain_opts *opts, void *stream);
+
 /* Same with host pointers; blocking (copies in, runs, copies out). */
 int fg_chain_host(int dtype, void *C, const void *B, long N, int num_gemm, const void *const *A, const long *M,
                   const long *K, const fg_chain_opts *opts);
@@ -156,10 +180,11 @@ int fg_conv_dims(const fg_conv_geom *g, long *K1, long *N);
 /* B = im2col(X) on device pointers: B is K1 x N row-major of the dtype. */
 int fg_im2col(int dtype, void *B, const void *X, const fg_conv_geom *g, void *stream);
 /* C = A_n ... A_1 im2col(X) on device pointers; K[0] must equal K1 of g and C is
- * M_n x N. The fused chain gathers its B tile from X; when the chain runs unfused
- * (options, LDS, the reference layout) B is formed in library scratch first. */
+ * M_n x N, epi as in fg_chain_epi (may be NULL). The fused chain gathers its B tile
+ * from X; when the chain runs unfused (options, LDS, the reference layout) B is
+ * formed in library scratch first. */
 int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
-                  const long *M, const long *K, const fg_chain_opts *opts, void *stream);
+                  const long *M, const long *K, const fg_epilogue *epi, const fg_chain_opts *opts, void *stream);
 
 /* ---- grouped launches (no reference counterpart: the reference runs one chain
  * per call). Many chains of uneven shapes and mixed dtypes, mathematical layout,

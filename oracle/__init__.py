@@ -185,3 +185,75 @@ def im2col(X: np.ndarray, kernel_h: int, kernel_w: int, stride=(1, 1), pad=(0, 0
             win = Xp[:, :, y0:y0 + sh * (OH - 1) + 1:sh, x0:x0 + sw * (OW - 1) + 1:sw]  # nb, C, OH, OW
             B[:, ky, kx] = win.transpose(1, 0, 2, 3)
     return B.reshape(C * kernel_h * kernel_w, nb * OH * OW)
+
+
+def _wrap32(a: np.ndarray) -> np.ndarray:
+    return ((np.asarray(a, np.int64) + (1 << 31)) % (1 << 32) - (1 << 31)).astype(np.int64)
+
+
+def epi_int8(acc: np.ndarray, bias=None, scale=None, zero_point: int = 0, relu: bool = False) -> np.ndarray:
+    """INT8 intermediate with the epilogue extensions (no reference counterpart; the
+    reference only saturates, multigemm/src/kernel_int8.h:329-385). Row m of acc is
+    output channel m: a = acc + bias[m] (int32 wrap); with a scale, t = rne(fp32(a) *
+    fp32 scale[m]) (half to even), else t = a; t is clamped to [-512, 512] (which
+    cannot change the result); q = sat8(t + zero_point); ReLU: max(q, zero_point)."""
+    a = _wrap32(np.asarray(acc, np.int64) + (0 if bias is None else np.asarray(bias, np.int64)[:, None]))
+    if scale is not None:
+        t = a.astype(np.float32) * np.asarray(scale, np.float32)[:, None]
+        t = np.rint(np.clip(t, np.float32(-512), np.float32(512))).astype(np.int64)
+    else:
+        t = np.clip(a, -512, 512)
+    q = np.clip(t + zero_point, -128, 127)
+    if relu:
+        q = np.maximum(q, zero_point)
+    return q.astype(np.int8)
+
+
+def epi_float(acc: np.ndarray, bias=None, scale=None, relu: bool = False) -> np.ndarray:
+    """FP32 / BF16 epilogue extensions on fp32 values: y = acc * scale[m], then
+    y + bias[m] (two fp32 roundings), then max(y, 0) under ReLU (NaN -> 0)."""
+    y = np.asarray(acc, np.float32)
+    if scale is not None:
+        y = (y * np.asarray(scale, np.float32)[:, None]).astype(np.float32)
+    if bias is not None:
+        y = (y + np.asarray(bias, np.float32)[:, None]).astype(np.float32)
+    if relu:
+        y = np.where(y > 0, y, np.float32(0)).astype(np.float32)
+    return y
+
+
+def chain_int8_epi(B: np.ndarray, As, epi: dict) -> np.ndarray:
+    """INT8 chain (mathematical layout) with epilogue extensions. epi holds per-GEMM
+    lists "bias", "scale", "zero_point", "relu" (None / 0 entries: not set). The GEMMs
+    are exact integer products u8(A) . s8(X) in int64, wrapped to int32; the last GEMM
+    gives int32 C = acc + bias[m], ReLU max(C, 0)."""
+    X = np.asarray(B).view(np.int8).astype(np.int64)
+    n = len(As)
+    get = lambda k, i, d=None: (epi.get(k) or [])[i] if i < len(epi.get(k) or []) else d
+    for i, A in enumerate(As):
+        acc = _wrap32(np.asarray(A).view(np.uint8).astype(np.int64) @ X)
+        b = get("bias", i)
+        if i == n - 1:
+            c = _wrap32(acc + (0 if b is None else np.asarray(b, np.int64)[:, None]))
+            if get("relu", i, 0):
+                c = np.maximum(c, 0)
+            return c.astype(np.int32)
+        X = epi_int8(acc, b, get("scale", i), int(get("zero_point", i, 0) or 0), bool(get("relu", i, 0))).astype(np.int64)
+    raise ValueError("empty chain")
+
+
+def chain_float_epi(B: np.ndarray, As, epi: dict, bf16: bool) -> np.ndarray:
+    """FP32 or BF16 chain (mathematical layout) with epilogue extensions, for
+    tolerance checks: each GEMM in float64 rounded to fp32 (the accumulator), the
+    extensions in fp32 (epi_float), BF16 intermediates rounded with bf16_rne."""
+    conv = (lambda x: bf16_to_f32(x).astype(np.float64)) if bf16 else (lambda x: np.asarray(x, np.float64))
+    X = conv(B)
+    n = len(As)
+    get = lambda k, i, d=None: (epi.get(k) or [])[i] if i < len(epi.get(k) or []) else d
+    for i, A in enumerate(As):
+        acc = (conv(A) @ X).astype(np.float32)
+        y = epi_float(acc, get("bias", i), get("scale", i), bool(get("relu", i, 0)))
+        if i == n - 1:
+            return y
+        X = bf16_to_f32(bf16_rne(y)).astype(np.float64) if bf16 else y.astype(np.float64)
+    raise ValueError("empty chain")

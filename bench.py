@@ -135,6 +135,23 @@ def bench_sgemm(torch, fg, timer, stream, M, N, K, mode, steps=20, warmup=3):
                 gbs=byts / ms / 1e6, frac_hbm=byts / ms / 1e6 / HBM_GBS, rel_err_vs_fp64=err)
 
 
+def bench_sgemm_t(torch, fg, timer, stream, M=2048, N=2048, K=2048, steps=20, warmup=3):
+    """FP32 GEMM with transposed operand storage (fg_sgemm_t: NT, TN, TT) beside NN;
+    the reference's dispatcher computes NN only (singlegemm/FP32_ID23.c:39-43)."""
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    A = uniform(torch, (M, K), gen)
+    B = uniform(torch, (K, N), gen)
+    At, Bt = A.t().contiguous(), B.t().contiguous()
+    C = torch.empty((M, N), device="cuda", dtype=torch.float32)
+    flop = 2.0 * M * N * K
+    out = {}
+    for ta, tb in ((0, 0), (0, 1), (1, 0), (1, 1)):
+        a, b = (At if ta else A), (Bt if tb else B)
+        ms = timer.run(lambda: fg.sgemm_t(ta, tb, C, a, b, M, N, K, stream=stream), steps, warmup)
+        out["NT"[ta] + "NT"[tb]] = dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / PEAK["fp32"])
+    return out
+
+
 def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
     """X(16384x512).W1(512x512).W2(512x1024).W3(1024x512) in the reference
     orientation: B = X^T (512 x 16384), A_i = W_i^T."""
@@ -414,6 +431,7 @@ def main() -> None:
     extra = (
         ("sgemm_2048_fp32_mfma", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_MFMA)),
         ("sgemm_2048_bf16x3", lambda: bench_sgemm(torch, fg, timer, stream, 2048, 2048, 2048, fg.F32_BF16X3)),
+        ("sgemm_2048_transposed", lambda: bench_sgemm_t(torch, fg, timer, stream)),
         ("sgemm_8192x64x8192", lambda: bench_sgemm(torch, fg, timer, stream, 8192, 64, 8192, fg.F32_MFMA)),
         ("sgemm_64x64x65536", lambda: bench_sgemm(torch, fg, timer, stream, 64, 64, 65536, fg.F32_MFMA)),
         ("chain3_int8_16384", lambda: bench_chain3(torch, fg, timer, stream, fg.DTYPE_INT8)),

@@ -118,6 +118,7 @@ def lib() -> ctypes.CDLL:
         L.fg_chain_epi.argtypes = [i, vp, vp, l, i, vpp, lp, lp, ctypes.POINTER(Epilogue),
                                    ctypes.POINTER(ChainOpts), vp]
         L.fg_sgemm.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
+        L.fg_sgemm_t.argtypes = [i, i, vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_host.argtypes = [vp, vp, vp, l, l, l, i]
         L.fg_f64_chain.argtypes = [vp, vp, l, i, vpp, lp, lp, vp]
         L.fg_to_f64.argtypes = [i, vp, vp, l, vp]
@@ -262,6 +263,13 @@ def chain_epi(dtype: int, C, B, As, epi: Epilogue | None, N: int | None = None, 
 
 def sgemm(C, A, B, M: int, N: int, K: int, f32_mode: int = F32_MFMA, splits: int = 0, stream=None) -> None:
     _check(lib().fg_sgemm(_addr(C), _addr(A), _addr(B), M, N, K, f32_mode, splits, _stream(stream)))
+
+
+def sgemm_t(transa: int, transb: int, C, A, B, M: int, N: int, K: int, f32_mode: int = F32_MFMA, splits: int = 0,
+            stream=None) -> None:
+    """C (M x N) = op(A) . op(B); transa: A stored K x M, transb: B stored N x K."""
+    _check(lib().fg_sgemm_t(transa, transb, _addr(C), _addr(A), _addr(B), M, N, K, f32_mode, splits,
+                            _stream(stream)))
 
 
 def f64_chain(C, B, As, N: int | None = None, Ms=None, Ks=None, stream=None) -> None:

@@ -107,13 +107,13 @@ int plan_splits(long M, long N, long K, int kind) {
 
 // Fused tile width: the widest tile that fits LDS and still gives every CU a
 // workgroup; if none gives enough workgroups, the narrowest that fits.
-int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K) {
+int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bool epi) {
   if (num_gemm > chain_fused_max_gemms() || !chain_fused_shape_ok(kind, N, num_gemm, K)) return 0;
   const int cand[4] = {128, 96, 64, 32};
   const int widest = chain_fused_max_tile(kind);
   int best = 0;
   for (int c : cand) {
-    if (c > widest || chain_fused_lds_bytes(kind, M, K, num_gemm, c) > 160 * 1024) continue;
+    if (c > widest || chain_fused_lds_bytes(kind, M, K, num_gemm, c, epi) > LDS_BYTES_MAX) continue;
     long tiles = (N + c - 1) / c;
     if (tiles >= num_cus()) return c;
     best = c;  // keep shrinking for more workgroups
@@ -230,8 +230,9 @@ static int ref_panels(long N, int threads, RefPanels *p) {
 
 // Decision order: explicit options, then the autotune table, then the heuristics
 // (widest tile that fits and still fills the CUs; split-K for skinny GEMMs).
+// epi: the chain has epilogue extensions (their per-row tables take LDS too).
 static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
-                      int *fused, int *tile_n, int *splits) {
+                      int *fused, int *tile_n, int *splits, bool epi = false) {
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
   int kind = kind_of(dtype, o.f32_mode);
@@ -241,7 +242,7 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
     int f, t, sp;
     if (autotune_lookup(kind, N, num_gemm, M, K, &f, &t, &sp) &&
         (!f || (chain_fused_shape_ok(kind, N, num_gemm, K) &&
-                chain_fused_lds_bytes(kind, M, K, num_gemm, t) <= LDS_BYTES_MAX))) {
+                chain_fused_lds_bytes(kind, M, K, num_gemm, t, epi) <= LDS_BYTES_MAX))) {
       *fused = f;
       *tile_n = f ? t : 0;
       *splits = f ? 0 : sp;
@@ -252,10 +253,10 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   if (!ref && o.fuse != FG_FUSE_OFF) {
     if (o.tile_n > 0) {
       if (o.tile_n <= chain_fused_max_tile(kind) && chain_fused_shape_ok(kind, N, num_gemm, K) &&
-          chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n) <= LDS_BYTES_MAX)
+          chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n, epi) <= LDS_BYTES_MAX)
         tn = o.tile_n;
     } else {
-      tn = plan_tile_n(kind, N, num_gemm, M, K);
+      tn = plan_tile_n(kind, N, num_gemm, M, K, epi);
     }
   }
   if (o.fuse == FG_FUSE_ON && tn == 0)
@@ -275,11 +276,12 @@ int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K,
 // One GEMM (+ INT8 column sums, + split-K reduction) of the unfused path. With
 // epilogue extensions (ex != nullptr) the GEMM always writes fp32 / int32 slabs and
 // the reduction pass applies the extensions while it forms the output.
+// trans (FP32 kinds): bit 0 A stored K x M, bit 1 B stored N x K.
 static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K, int splits,
-                    hipStream_t s, const EpiOne *ex = nullptr) {
+                    hipStream_t s, const EpiOne *ex = nullptr, int trans = 0) {
   int *colsum = nullptr;
   int rc;
-  if (kind == K_F32 && epi != EPI_PARTIAL && splits <= 0 && skinny_f32_shape(M, N)) {
+  if (kind == K_F32 && epi != EPI_PARTIAL && splits <= 0 && trans == 0 && skinny_f32_shape(M, N)) {
     // small M or N: 64 x 64 tiles, split-K over the grid
     long kper = 0;
     const int sk = skinny_f32_splits(M, N, K, &kper);
@@ -293,7 +295,7 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
   }
   if (splits <= 0) splits = plan_splits(M, N, K, kind);
   if (splits <= 1 && !ex) {  // INT8: the kernel sums its own B columns (colsum == nullptr)
-    GemmArgs g{kind, epi, A, B, C, nullptr, M, N, K, 1};
+    GemmArgs g{kind, epi, A, B, C, nullptr, M, N, K, 1, trans};
     return hip_status(launch_gemm(g, s), "gemm");
   }
   if (kind == K_I8) {  // split-K slabs see part of K: the reduction adds the full column sums
@@ -302,7 +304,7 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
   }
   void *part = nullptr;
   if ((rc = scratch(s, "partials", (size_t)splits * M * N * 4, &part))) return rc;
-  GemmArgs g{kind, EPI_PARTIAL, A, B, part, colsum, M, N, K, splits};
+  GemmArgs g{kind, EPI_PARTIAL, A, B, part, colsum, M, N, K, splits, trans};
   if ((rc = hip_status(launch_gemm(g, s), "gemm(split-K)"))) return rc;
   // the kernel may use fewer slabs than asked when K is short
   long bk = (kind == K_I8) ? 128 : (kind == K_BF16 ? 64 : 32);
@@ -411,7 +413,7 @@ static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, c
 
   int fused = 0, tn = 0;
   int splits = 0;
-  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits))) return rc;
+  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr))) return rc;
   if (fused) {
     ChainArgs c = chain_args(kind, B, C, N, num_gemm, A, M, K, tn);
     c.epi = ep;
@@ -466,6 +468,18 @@ int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, i
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
   int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
   return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream);
+}
+
+int fg_sgemm_t(int transa, int transb, float *C, const float *A, const float *B, long M, long N, long K,
+               int f32_mode, int splits, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
+    return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
+  if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1))
+    return set_error(FG_ERR_INVALID, "sgemm: transa=%d transb=%d (0 or 1)", transa, transb);
+  int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, transa | (transb << 1));
 }
 
 // ------------------------------------------------------- convolution front end --
@@ -548,7 +562,7 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
   // The gathering kernel loads single elements, so the fused path takes any N: plan
   // with N rounded up to whole 4-column groups.
   int fused = 0, tn = 0, splits = 0;
-  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &fused, &tn, &splits))) return rc;
+  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr))) return rc;
   if (fused) {
     ChainArgs c = chain_args(kind, X, C, N, num_gemm, A, M, K, tn);
     c.img = &ig;

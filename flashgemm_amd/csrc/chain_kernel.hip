@@ -414,8 +414,19 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   const int M = (int)P.M[gi], Mp = (M + 31) & ~31;
   const int rb0 = (int)rb0_l;
   const EpiOne ex = EPX ? P.epi.g[gi] : EpiOne{};
-  const int *const bias_i = (const int *)ex.bias;
-  const float *const bias_f = (const float *)ex.bias;
+  // per-row bias bits and scales of GEMM gi in LDS (filled at kernel start)
+  const uint32_t *lb = nullptr;
+  const float *ls = nullptr;
+  if (EPX) {
+    int rows = 0, off = 0;
+    for (int g = 0; g < P.num_gemm; ++g) {
+      off += g < gi ? (int)P.M[g] : 0;
+      rows += (int)P.M[g];
+    }
+    lb = (const uint32_t *)((const char *)arena + P.epi_off) + off;
+    ls = (const float *)((const uint32_t *)((const char *)arena + P.epi_off) + rows) + off;
+  }
+  const bool has_bias = ex.bias != nullptr, has_scale = ex.scale != nullptr;
   if (gi == P.num_gemm - 1) {
     const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
     const size_t rs = (size_t)P.N * 4;  // bytes per row of C
@@ -436,13 +447,11 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
           if (full || m < M) {
             if (KIND == K_I8) {
               int v = (int)acc[i][j][r];
-              if (EPX) v = epi_i32(v, bias_i ? bias_i[m] : 0, ex.relu);
+              if (EPX) v = epi_i32(v, (int)lb[m], ex.relu);
               *(int *)(base + ro * rs) = v;
             } else {
               float v = (float)acc[i][j][r];
-              if (EPX)
-                v = epi_f(v, ex.scale != nullptr, ex.scale ? ex.scale[m] : 1.0f, bias_f != nullptr,
-                          bias_f ? bias_f[m] : 0.0f, ex.relu);
+              if (EPX) v = epi_f(v, has_scale, ls[m], has_bias, __builtin_bit_cast(float, lb[m]), ex.relu);
               *(float *)(base + ro * rs) = v;
             }
           }
@@ -481,9 +490,9 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int m = m0r + e < M ? m0r + e : M - 1;
-          eb[e] = (KIND == K_I8 && bias_i) ? bias_i[m] : 0;
-          ef[e] = (KIND != K_I8 && bias_f) ? bias_f[m] : 0.0f;
-          es[e] = ex.scale ? ex.scale[m] : 1.0f;
+          eb[e] = (int)lb[m];
+          ef[e] = __builtin_bit_cast(float, lb[m]);
+          es[e] = ls[m];
         }
       }
 #pragma unroll
@@ -494,7 +503,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             a4[e] = (int)acc[i][j][4 * q + e];
-            if (EPX) a4[e] = epi_i8(a4[e], eb[e], ex.scale != nullptr, es[e], ex.zp, ex.relu);
+            if (EPX) a4[e] = epi_i8(a4[e], eb[e], has_scale, es[e], ex.zp, ex.relu);
           }
           uint32_t w = sat_pk_i8(a4[0], a4[1], a4[2], a4[3]);
           if (!full) w &= valid == 0 ? 0u : (0xffffffffu >> (8 * (4 - valid)));
@@ -505,7 +514,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             float a = (float)acc[i][j][4 * q + e];
-            if (EPX) a = epi_f(a, ex.scale != nullptr, es[e], bias_f != nullptr, ef[e], ex.relu);
+            if (EPX) a = epi_f(a, has_scale, es[e], has_bias, ef[e], ex.relu);
             v[e] = (full || e < valid) ? a : 0.0f;
           }
           if (KIND == K_F32) {
@@ -611,6 +620,25 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   // always exists: the loads stay unconditional and the wait counts after them exact.
   stage_b<KIND, TN, IMG>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img,
                          [&] { load_chunk(apk ? apk : (const char *)P.A[0], fa[0]); });
+  if (EPX) {
+    // every GEMM's per-row bias and scale into LDS (bias bits, then scales, rows of
+    // GEMM g after those of GEMMs < g): the epilogues read them with LDS loads, so
+    // they never wait behind the weight stream's global loads
+    uint32_t *const eb = (uint32_t *)((char *)arena + P.epi_off);
+    int rows = 0;
+    for (int g = 0; g < P.num_gemm; ++g) rows += (int)P.M[g];
+    float *const es = (float *)(eb + rows);
+    int off = 0;
+    for (int g = 0; g < P.num_gemm; ++g) {
+      const uint32_t *bsrc = (const uint32_t *)P.epi.g[g].bias;
+      const float *ssrc = P.epi.g[g].scale;
+      for (int t = fresh_tid(); t < (int)P.M[g]; t += NTH) {
+        eb[off + t] = bsrc ? bsrc[t] : 0u;
+        es[off + t] = ssrc ? ssrc[t] : 1.0f;
+      }
+      off += (int)P.M[g];
+    }
+  }
   __syncthreads();
 
   // rows [Mp, Mc) of Y_gi (the k-padding of the next GEMM) must read as zero
@@ -789,7 +817,8 @@ size_t arena_bytes(int kind, int S, int tile_n) {
 // it is done (*hold = 0). Otherwise the arena shrinks to what the held final passes
 // need (*hold = 1): all rows of Y_i but each wave's final pass are written while X_i
 // is still read, so they must not meet it: Kc_i + (passes_i - 1) * PASS <= S.
-int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *xbase, int *ybase, int *hold) {
+int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *xbase, int *ybase, int *hold,
+               size_t budget = LDS_LIMIT) {
   const long cs = CS_of(kind);
   auto rc = [cs](long v) { return (v + cs - 1) / cs * cs; };
   auto r32 = [](long v) { return (v + 31) / 32 * 32; };
@@ -805,7 +834,7 @@ int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *x
       wide = std::max(wide, kc + rc(M[i]));
     }
   }
-  *hold = tile_n <= 0 || arena_bytes(kind, (int)wide, tile_n) > LDS_LIMIT;
+  *hold = tile_n <= 0 || arena_bytes(kind, (int)wide, tile_n) > budget;
   if (!*hold) S = wide;
   long x = 0;
   for (int i = 0; i < n; ++i) {
@@ -823,7 +852,7 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   make_chain_params(c, &P);
   if (!c.img && !chain_fused_aligned(P)) return hipErrorInvalidValue;
   constexpr int TN = TNB * 32;
-  const size_t lds = arena_bytes(KIND, P.S, TN);
+  const size_t lds = c.epi ? (size_t)P.epi_off + chain_epi_lds_bytes(c.M, c.num_gemm) : arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
   auto kfn = c.img ? (c.epi ? chain_kernel<KIND, TNB, true, true> : chain_kernel<KIND, TNB, true, false>)
                    : (c.epi ? chain_kernel<KIND, TNB, false, true> : chain_kernel<KIND, TNB, false, false>);
@@ -891,12 +920,20 @@ hipError_t launch_pack_t(PackJob J, hipStream_t s) {
 
 }  // namespace
 
-size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n) {
+size_t chain_epi_lds_bytes(const long *M, int num_gemm) {
+  long rows = 0;
+  for (int i = 0; i < num_gemm; ++i) rows += M[i];
+  return (size_t)rows * 8;
+}
+
+size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n, bool epi) {
   int xb[MAXG], yb[MAXG];
   if (num_gemm < 1 || num_gemm > MAXG) return (size_t)-1;
   int hold;
-  const int S = plan_arena(kind, M, K, num_gemm, tile_n, xb, yb, &hold);
-  return arena_bytes(kind, S, tile_n);
+  const size_t extra = epi ? chain_epi_lds_bytes(M, num_gemm) : 0;
+  if (extra >= LDS_LIMIT) return (size_t)-1;
+  const int S = plan_arena(kind, M, K, num_gemm, tile_n, xb, yb, &hold, LDS_LIMIT - extra);
+  return (arena_bytes(kind, S, tile_n) + 15) / 16 * 16 + extra;
 }
 
 int chain_fused_max_gemms() { return MAXG; }
@@ -929,7 +966,8 @@ hipError_t launch_pack_a(int kind, const PackJob &J, hipStream_t s) {
 
 hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s) {
   if (c.num_gemm < 1 || c.num_gemm > MAXG || c.N <= 0) return hipErrorInvalidValue;
-  if (chain_fused_lds_bytes(c.kind, c.M, c.K, c.num_gemm, c.tile_n) > LDS_LIMIT) return hipErrorInvalidValue;
+  if (chain_fused_lds_bytes(c.kind, c.M, c.K, c.num_gemm, c.tile_n, c.epi != nullptr) > LDS_LIMIT)
+    return hipErrorInvalidValue;
   switch (c.kind) {
     case K_F32: return launch_chain_k<K_F32>(c, s);
     case K_BF16: return launch_chain_k<K_BF16>(c, s);
@@ -950,7 +988,10 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
     P->M[i] = c.M[i];
     P->K[i] = c.K[i];
   }
-  P->S = plan_arena(c.kind, c.M, c.K, c.num_gemm, c.tile_n, P->xbase, P->ybase, &P->hold);
+  const size_t extra = c.epi ? chain_epi_lds_bytes(c.M, c.num_gemm) : 0;
+  P->S = plan_arena(c.kind, c.M, c.K, c.num_gemm, c.tile_n, P->xbase, P->ybase, &P->hold,
+                    extra < LDS_LIMIT ? LDS_LIMIT - extra : 0);
+  P->epi_off = (int)((arena_bytes(c.kind, P->S, c.tile_n) + 15) / 16 * 16);
   P->B = c.B;
   P->C = c.C;
   P->N = c.N;

@@ -188,6 +188,7 @@ struct GemmArgs {
   const int *colsum;// INT8 split-K: sum_k B[k][n] (int32, N entries); nullptr: the kernel sums its panel
   long M, N, K;
   int splits;       // split-K factor (EPI_PARTIAL writes splits slabs of M x N)
+  int trans;        // FP32 kinds, EPI_STORE / EPI_PARTIAL: bit 0 A stored K x M, bit 1 B stored N x K
 };
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);
@@ -249,6 +250,7 @@ struct ChainParams {
   int vecB;
   ImgGeom img;              // geometry when B is gathered from an image (chain_kernel IMG)
   EpiParams epi;            // epilogue extensions (chain_kernel EPX)
+  int epi_off;              // EPX: LDS byte offset of the per-row bias and scale tables
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);
@@ -273,7 +275,10 @@ hipError_t launch_pack_a(int kind, const PackJob &J, hipStream_t s);
 // Fused chain: every intermediate stays in LDS. Returns hipErrorInvalidValue when the
 // largest (K_i + M_i) x tile_n panel does not fit the LDS budget.
 hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s);
-size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n);
+// LDS of a fused launch; epi: with the per-row bias and scale tables of the epilogue
+// extensions (chain_epi_lds_bytes: 8 bytes per row of every GEMM).
+size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n, bool epi = false);
+size_t chain_epi_lds_bytes(const long *M, int num_gemm);
 int chain_fused_max_gemms();
 int chain_fused_max_tile(int kind);  // widest fused tile_n for the kind
 constexpr size_t LDS_BYTES_MAX = 160 * 1024;

@@ -14,7 +14,7 @@ int num_cus();
 int scratch(hipStream_t s, const char *name, size_t bytes, void **out);
 int kind_of(int dtype, int f32_mode);
 int plan_splits(long M, long N, long K, int kind);
-int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K);
+int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bool epi = false);
 // Autotune table lookup (autotune.cpp): true when the shape has a measured decision.
 bool autotune_lookup(int kind, long N, int n, const long *M, const long *K, int *fused, int *tile_n, int *splits);
 

@@ -78,9 +78,19 @@ struct Stager {
   uint32_t a[16];  // 4 x 16 B of A
   uint32_t b[16];  // RB rows x CW columns of B
 
-  // A tile rows [m0, m0+BT) x k [k0, k0+BK): 8 BT 16-byte units, 4 per thread.
-  __device__ __forceinline__ void load_a(const char *A, long M, long K, long kend, long m0, long k0,
-                                         bool vec, int t) {
+  // Operands are staged in one of two ways, by their memory layout:
+  //   rows: an operand whose k runs along memory rows (A as stored M x K, or a
+  //     transposed B stored N x K): 16-byte k-units per row, load_rows / store_rows;
+  //   cols: an operand whose k runs down the columns (B as stored K x N, or a
+  //     transposed A stored K x M): RB rows x CW columns per thread, transposed into
+  //     k-groups on the way to LDS, load_cols / store_cols.
+  // Transposed operands are supported for the FP32 kinds (no INT8 u8 bias or column
+  // sums on those paths).
+  //
+  // Rows [m0, m0+BT) x k [k0, k0+BK) of a row-major R x K operand: 8 BT 16-byte units,
+  // 4 per thread (INT8: the u8 -> s8 bias of A is applied here).
+  __device__ __forceinline__ void load_rows(const char *A, long M, long K, long kend, long m0, long k0,
+                                            bool vec, int t, uint32_t (&a)[16]) {
     if (vec && m0 + BT <= M && k0 + T::BK <= kend) {  // interior tile (uniform branch)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -129,7 +139,7 @@ struct Stager {
     }
   }
 
-  __device__ __forceinline__ void store_a(uint4 *sA, int t) const {
+  __device__ __forceinline__ void store_rows(uint4 *sA, int t, const uint32_t (&a)[16]) const {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       int u = t + NT * q;
@@ -150,9 +160,10 @@ struct Stager {
     }
   }
 
-  // B tile k [k0, k0+BK) x columns [n0, n0+BT): thread = RB rows x CW columns.
-  __device__ __forceinline__ void load_b(const char *B, long N, long kend, long n0, long k0, bool vec,
-                                         int t) {
+  // k [k0, k0+BK) x columns [n0, n0+BT) of a row-major K x N operand: thread = RB
+  // rows x CW columns.
+  __device__ __forceinline__ void load_cols(const char *B, long N, long kend, long n0, long k0, bool vec,
+                                            int t, uint32_t (&b)[16]) {
     constexpr int NCB = BT / T::CW;  // column blocks
     int cb = t % NCB, kg = t / NCB;
     long n = n0 + (long)cb * T::CW;
@@ -214,7 +225,7 @@ struct Stager {
   // v_perm_b32) for INT8, one v_perm_b32 per pair of rows for BF16.
   // INT8 also adds the s8 sums of the 4 columns it stages into cs (the column sums of
   // the workgroup's B panel, for the u8 bias).
-  __device__ __forceinline__ void store_b(uint4 *sB, int t, int (&cs)[4]) const {
+  __device__ __forceinline__ void store_cols(uint4 *sB, int t, int (&cs)[4], const uint32_t (&b)[16]) const {
     constexpr int NCB = BT / T::CW;
     int cb = t % NCB, g = t / NCB;
     uint32_t wi[4][4];  // INT8: wi[c][d] = rows 4d..4d+3 of column c
@@ -331,7 +342,9 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &
   tn = L / tiles_m;
 }
 
-template <int KIND, int EPI, class TL>
+// TRANS bit 0: A is stored transposed (K x M, op(A) = A^T); bit 1: B is stored
+// transposed (N x K). FP32 kinds only.
+template <int KIND, int EPI, class TL, int TRANS>
 __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
                                                       void *__restrict__ Cv, const int *__restrict__ colsum,
                                                       long M, long N, long K, long kper, int tiles_m, int tiles_n,
@@ -358,31 +371,36 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
 
   const char *A = (const char *)Av;
   const char *B = (const char *)Bv;
+  static_assert(TRANS == 0 || KIND == K_F32 || KIND == K_F32X3, "transposed operands: FP32 kinds only");
+  constexpr bool TA = TRANS & 1, TB = (TRANS >> 1) & 1;
   Stager<KIND, TL> st;
-  int cs[4] = {0, 0, 0, 0};
+  int cs[4] = {0, 0, 0, 0}, cs_unused[4];
+  auto load = [&](long k0) {
+    if (TA) st.load_cols(A, M, kend, m0, k0, vecA, t, st.a);
+    else st.load_rows(A, M, K, kend, m0, k0, vecA, t, st.a);
+    if (TB) st.load_rows(B, N, K, kend, n0, k0, vecB, t, st.b);
+    else st.load_cols(B, N, kend, n0, k0, vecB, t, st.b);
+  };
+  auto store = [&](uint4 *dst) {
+    if (TA) st.store_cols(dst, t, cs_unused, st.a);
+    else st.store_rows(dst, t, st.a);
+    if (TB) st.store_rows(dst + TILE_U4, t, st.b);
+    else st.store_cols(dst + TILE_U4, t, cs, st.b);
+  };
   int nstages = (int)((kend - kbeg + T::BK - 1) / T::BK);
   if (nstages > 0) {
-    st.load_a(A, M, K, kend, m0, kbeg, vecA, t);
-    st.load_b(B, N, kend, n0, kbeg, vecB, t);
-    st.store_a(smem, t);
-    st.store_b(smem + TILE_U4, t, cs);
+    load(kbeg);
+    store(smem);
   }
   __syncthreads();
   for (int s = 0; s < nstages; ++s) {
     uint4 *cur = smem + (s & 1) * 2 * TILE_U4;
     uint4 *nxt = smem + ((s + 1) & 1) * 2 * TILE_U4;
     const bool more = s + 1 < nstages;
-    if (more) {
-      long k0 = kbeg + (long)(s + 1) * T::BK;
-      st.load_a(A, M, K, kend, m0, k0, vecA, t);
-      st.load_b(B, N, kend, n0, k0, vecB, t);
-    }
+    if (more) load(kbeg + (long)(s + 1) * T::BK);
 #pragma unroll
     for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND, TL>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
-    if (more) {
-      st.store_a(nxt, t);
-      st.store_b(nxt + TILE_U4, t, cs);
-    }
+    if (more) store(nxt);
     __syncthreads();
   }
   // INT8 without a column-sum array (no split-K): the workgroup staged every row of its
@@ -569,7 +587,7 @@ __global__ void ref_permute_kernel(const E *__restrict__ Z, E *__restrict__ X, l
   X[e] = Z[(size_t)r * p.N + p0 + c];
 }
 
-template <int KIND, int EPI, class TL>
+template <int KIND, int EPI, class TL, int TRANS>
 hipError_t launch_tile(const GemmArgs &g, hipStream_t s) {
   using T = KT<KIND>;
   int tiles_m = (int)((g.M + TL::BT - 1) / TL::BT), tiles_n = (int)((g.N + TL::BT - 1) / TL::BT);
@@ -577,10 +595,15 @@ hipError_t launch_tile(const GemmArgs &g, hipStream_t s) {
   long kper = ((g.K + splits - 1) / splits + T::BK - 1) / T::BK * T::BK;
   splits = (int)((g.K + kper - 1) / kper);
   int esz = T::ESZ;
-  bool vecA = ((g.K * esz) % 16 == 0) && (((uintptr_t)g.A & 15) == 0);
-  bool vecB = (g.N % T::CW == 0) && (((uintptr_t)g.B & (T::CW * esz - 1)) == 0);
+  // vector loads: 16-byte k-units along rows (rows layout) or CW-element pieces of a row (cols layout)
+  auto vec_rows = [&](const void *p, long ld) { return (ld * esz) % 16 == 0 && ((uintptr_t)p & 15) == 0; };
+  auto vec_cols = [&](const void *p, long ld) {
+    return ld % T::CW == 0 && ((uintptr_t)p & (T::CW * esz - 1)) == 0;
+  };
+  bool vecA = (TRANS & 1) ? vec_cols(g.A, g.M) : vec_rows(g.A, g.K);
+  bool vecB = (TRANS & 2) ? vec_rows(g.B, g.K) : vec_cols(g.B, g.N);
   const size_t lds = 4 * (size_t)TL::TU4 * sizeof(uint4);
-  auto kfn = gemm_kernel<KIND, EPI, TL>;
+  auto kfn = gemm_kernel<KIND, EPI, TL, TRANS>;
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid(tiles_m * tiles_n, splits);
@@ -591,11 +614,26 @@ hipError_t launch_tile(const GemmArgs &g, hipStream_t s) {
 
 // The 256 x 256 tile reads half the operand bytes per MFMA of the 128 x 128 tile;
 // it is used when it still gives about one tile per CU.
+template <int KIND, int EPI, int TRANS>
+hipError_t launch_trans(const GemmArgs &g, hipStream_t s) {
+  const long big = ((g.M + 255) / 256) * ((g.N + 255) / 256) * (g.splits > 1 ? g.splits : 1);
+  if (big >= 192) return launch_tile<KIND, EPI, TileL, TRANS>(g, s);
+  return launch_tile<KIND, EPI, TileS, TRANS>(g, s);
+}
+
 template <int KIND, int EPI>
 hipError_t launch_one(const GemmArgs &g, hipStream_t s) {
-  const long big = ((g.M + 255) / 256) * ((g.N + 255) / 256) * (g.splits > 1 ? g.splits : 1);
-  if (big >= 192) return launch_tile<KIND, EPI, TileL>(g, s);
-  return launch_tile<KIND, EPI, TileS>(g, s);
+  if constexpr (KIND == K_F32 || KIND == K_F32X3) {
+    if constexpr (EPI != EPI_REQUANT) {  // transposed operands: single FP32 GEMMs (C or split-K slabs)
+      switch (g.trans) {
+        case 1: return launch_trans<KIND, EPI, 1>(g, s);
+        case 2: return launch_trans<KIND, EPI, 2>(g, s);
+        case 3: return launch_trans<KIND, EPI, 3>(g, s);
+      }
+    }
+  }
+  if (g.trans != 0) return hipErrorInvalidValue;
+  return launch_trans<KIND, EPI, 0>(g, s);
 }
 
 template <int KIND>

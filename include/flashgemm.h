@@ -136,6 +136,13 @@ int fg_chain_host(int dtype, void *C, const void *B, long N, int num_gemm, const
 int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode, int splits,
              void *stream);
 int fg_sgemm_host(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode);
+/* C = op(A) . op(B) with op(A) M x K and op(B) K x N: transa = 1 reads A stored
+ * K x M (row-major, lda = M), transb = 1 reads B stored N x K (ldb = K); C is M x N.
+ * The reference's dispatcher takes the same flags but computes NN only (NT is an
+ * empty stub, singlegemm/FP32_ID23.c:39-43); LibShalom_sgemm_mp keeps that
+ * behaviour, this call computes every combination. */
+int fg_sgemm_t(int transa, int transb, float *C, const float *A, const float *B, long M, long N, long K,
+               int f32_mode, int splits, void *stream);
 
 /* Weights in the fused kernels' packed layout: MFMA fragment order, zero padded,
  * INT8 bytes pre-biased to signed, FP32 for FG_F32_BF16X3 pre-split into bf16 hi/lo.

@@ -105,18 +105,40 @@ int plan_splits(long M, long N, long K, int kind) {
   return (int)(s < 1 ? 1 : s);
 }
 
-// Fused tile width: the widest tile that fits LDS and still gives every CU a
-// workgroup; if none gives enough workgroups, the narrowest that fits.
+// Estimated cycles of a fused plan on one CU. Tiles run in rounds of one per CU. A
+// tile costs about 10^4 cycles of staging, barriers and epilogues plus the MFMA
+// issue of its busiest wave: passes_i x k-steps_i x (row blocks x column blocks per
+// wave) MFMA groups per GEMM, at the MFMA's cycles per group times the two waves
+// that share a SIMD (INT8/BF16: measured ~1.5x the MFMA rate with the weight stream).
+static double fused_cost(int kind, long N, int num_gemm, const long *M, const long *K, int tn) {
+  const int groups = chain_fused_wave_groups(kind, tn);
+  const int rows = chain_fused_rows_per_wave(kind);
+  const long pass = (long)(8 / groups) * rows;
+  const int blocks = (rows / 32) * (tn / 32 / groups);
+  const long kc = kind == K_F32 ? 8 : (kind == K_I8 ? 32 : 16);
+  const double unit = kind == K_F32 ? 4 * 64 * 2 : (kind == K_F32X3 ? 3 * 32 * 2 : 32 * 2 * 1.5);
+  double t = 0;
+  for (int i = 0; i < num_gemm; ++i)
+    t += (double)((M[i] + pass - 1) / pass) * ((K[i] + kc - 1) / kc) * blocks * unit;
+  const long tiles = (N + tn - 1) / tn;
+  const long rounds = (tiles + num_cus() - 1) / num_cus();
+  return (double)rounds * (t + 1e4);
+}
+
+// Fused tile width: the cheapest tile (fused_cost) that fits LDS; within 3% the one
+// with more workgroups. 0 when none fits.
 int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bool epi) {
   if (num_gemm > chain_fused_max_gemms() || !chain_fused_shape_ok(kind, N, num_gemm, K)) return 0;
-  const int cand[4] = {128, 96, 64, 32};
-  const int widest = chain_fused_max_tile(kind);
   int best = 0;
-  for (int c : cand) {
-    if (c > widest || chain_fused_lds_bytes(kind, M, K, num_gemm, c, epi) > LDS_BYTES_MAX) continue;
-    long tiles = (N + c - 1) / c;
-    if (tiles >= num_cus()) return c;
-    best = c;  // keep shrinking for more workgroups
+  double best_cost = 0;
+  for (int c : {32, 64, 96, 128, 256}) {  // narrow first: ties keep more workgroups
+    if (!chain_fused_tile_ok(kind, c) || chain_fused_lds_bytes(kind, M, K, num_gemm, c, epi) > LDS_BYTES_MAX)
+      continue;
+    const double cost = fused_cost(kind, N, num_gemm, M, K, c);
+    if (best == 0 || cost < best_cost * 0.97) {
+      best = c;
+      best_cost = cost;
+    }
   }
   return best;
 }
@@ -252,7 +274,7 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   int tn = 0;
   if (!ref && o.fuse != FG_FUSE_OFF) {
     if (o.tile_n > 0) {
-      if (o.tile_n <= chain_fused_max_tile(kind) && chain_fused_shape_ok(kind, N, num_gemm, K) &&
+      if (chain_fused_tile_ok(kind, o.tile_n) && chain_fused_shape_ok(kind, N, num_gemm, K) &&
           chain_fused_lds_bytes(kind, M, K, num_gemm, o.tile_n, epi) <= LDS_BYTES_MAX)
         tn = o.tile_n;
     } else {

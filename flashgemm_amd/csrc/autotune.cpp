@@ -98,8 +98,8 @@ int fg_autotune_chain(int dtype, int f32_mode, long N, int num_gemm, const long 
   Choice best{0, 0, 0, 1e30};
   struct Cand { int fuse, tile_n, splits; };
   std::vector<Cand> cands;
-  for (int tn : {32, 64, 96, 128})
-    if (tn <= chain_fused_max_tile(kind) && chain_fused_lds_bytes(kind, M, K, num_gemm, tn) <= LDS_BYTES_MAX)
+  for (int tn : {32, 64, 96, 128, 256})
+    if (chain_fused_tile_ok(kind, tn) && chain_fused_lds_bytes(kind, M, K, num_gemm, tn) <= LDS_BYTES_MAX)
       cands.push_back({FG_FUSE_ON, tn, 0});
   cands.push_back({FG_FUSE_OFF, 0, 1});
   cands.push_back({FG_FUSE_OFF, 0, 0});  // planner's split-K

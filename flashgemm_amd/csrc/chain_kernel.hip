@@ -400,11 +400,11 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
 // only evaluated in the block that straddles M. EPX applies GEMM gi's epilogue
 // extensions (P.epi.g[gi]: per-row bias and scale, zero point, ReLU; epi_i8, epi_i32,
 // epi_f); rows past M read no bias or scale.
-template <int KIND, int TNB, bool EPX>
+template <int KIND, int TN, int TNW, bool EPX>
 __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, long rb0_l, long n0, uint4 *arena,
-                                                const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB], int *csum) {
+                                                const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNW], int *csum,
+                                                int cb0) {
   using T = CT<KIND>;
-  constexpr int TN = TNB * 32;
   constexpr int RP = T::RP;
   const int S = P.S;
   const int SG = S / T::KG;
@@ -436,8 +436,8 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
       if (rbase >= Mp) continue;
       const bool full = rbase + 32 <= M;
 #pragma unroll
-      for (int j = 0; j < TNB; ++j) {
-        const int n = j * 32 + l32;
+      for (int j = 0; j < TNW; ++j) {
+        const int n = (cb0 + j) * 32 + l32;
         if (n >= tile_w) continue;
         char *base = (char *)P.C + ((size_t)(rbase + 4 * h) * P.N + n0 + n) * 4;
 #pragma unroll
@@ -468,9 +468,9 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   // Row r of the block lies in group r / KG at byte (r % KG) * EB of its slot.
   constexpr int EB = (KIND == K_F32) ? 4 : (KIND == K_I8 ? 1 : 2);  // arena bytes per element (per plane)
   const int lane_off = l32 * 16 + (T::KG == 4 ? h * TN * 16 : 4 * h * EB);
-  int cs[TNB];
+  int cs[TNW];
 #pragma unroll
-  for (int j = 0; j < TNB; ++j) cs[j] = 0;
+  for (int j = 0; j < TNW; ++j) cs[j] = 0;
 #pragma unroll
   for (int i = 0; i < RP; ++i) {
     const int rbase = rb0 + i * 32;
@@ -496,8 +496,8 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
         }
       }
 #pragma unroll
-      for (int j = 0; j < TNB; ++j) {
-        char *const dst = blk + ((8 * q / T::KG) * TN + j * 32) * 16 + (8 * q % T::KG) * EB;
+      for (int j = 0; j < TNW; ++j) {
+        char *const dst = blk + ((8 * q / T::KG) * TN + (cb0 + j) * 32) * 16 + (8 * q % T::KG) * EB;
         if (KIND == K_I8) {
           int a4[4];
 #pragma unroll
@@ -535,7 +535,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   }
   if (KIND == K_I8)
 #pragma unroll
-    for (int j = 0; j < TNB; ++j) atomicAdd(&csum[j * 32 + l32], cs[j]);  // integer: order independent
+    for (int j = 0; j < TNW; ++j) atomicAdd(&csum[(cb0 + j) * 32 + l32], cs[j]);  // integer: order independent
 }
 
 // One workgroup runs the whole chain on columns [n0, n0 + TN) of B. P.A[i] are
@@ -551,15 +551,24 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 // buffered a whole chunk ahead: while chunk c's MFMAs run, chunk c+1's weight
 // loads (global) and B reads (LDS) are in flight. The weight stream continues
 // across row passes and GEMM boundaries.
-template <int KIND, int TNB, bool IMG = false, bool EPX = false>
+//
+// G > 1 splits the waves into G column groups: wave w owns the TNB / G column blocks
+// starting at block (w % G) * TNB / G and walks the rows of every GEMM with the
+// NW / G waves of its group. Wide tiles with G = TNB (one 32-column block per wave)
+// keep every wave busy in chains whose M_i are a few row blocks, where G = 1 would
+// leave most waves without rows.
+template <int KIND, int TNB, bool IMG = false, bool EPX = false, int G = 1>
 __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
+  static_assert(TNB % G == 0 && NW % G == 0, "column groups must split the tile and the waves");
   constexpr int TN = TNB * 32;
+  constexpr int TNW = TNB / G;               // column blocks per wave
+  constexpr int NWR = NW / G;                // waves per column group
   constexpr int RP = T::RP;
   constexpr int PF = T::PF;
   constexpr int FRAG = 64 * T::FS;           // bytes per packed fragment
-  constexpr long PASS = (long)NW * RP * 32;  // rows of one pass of all waves
+  constexpr long PASS = (long)NWR * RP * 32;  // rows of one pass of a column group
   constexpr long CS = (long)PF * T::KC;      // rows of one chunk
   const int SG = P.S / T::KG;                // arena k-groups
   const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;  // lo plane (F32X3), in slots
@@ -569,7 +578,9 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   int *colsum = (int *)(arena + (size_t)SG * TN * (KIND == K_F32X3 ? 2 : 1));
 
   const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+  const int wave_id = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+  const int wave = wave_id / G;           // row position inside the column group
+  const int cb0 = (wave_id % G) * TNW;    // first column block of this wave
   const int h = lane >> 5, l32 = lane & 31;
 
   // first packed-weight chunk of this wave in GEMM gi (nullptr: no rows there);
@@ -599,11 +610,11 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
 
   // B fragments of one k-step: column block j of this lane at slot
   // (g0 + 2 s + h) * TN + 32 j + l32 for step s of the chunk whose first group is g0
-  const uint4 *lane_b = arena + h * TN + l32;
-  auto read_b = [&](int g0, int s, BFrag<KIND, TNB> &dst) {
+  const uint4 *lane_b = arena + h * TN + cb0 * 32 + l32;
+  auto read_b = [&](int g0, int s, BFrag<KIND, TNW> &dst) {
     const uint4 *p = lane_b + (g0 + 2 * s) * TN;
 #pragma unroll
-    for (int j = 0; j < TNB; ++j) {
+    for (int j = 0; j < TNW; ++j) {
       dst.b[j] = p[32 * j];
       if (KIND == K_F32X3) dst.lo[j] = p[lo_off + 32 * j];
     }
@@ -662,7 +673,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     const int kspad = (int)kspad_of<KIND>(K);
     const int nchunks = kspad / PF;
     const int ksteps = (int)((K + T::KC - 1) / T::KC);  // k-steps with real data
-    const size_t pass_stride = (size_t)NW * kspad * RP * FRAG;
+    const size_t pass_stride = (size_t)NWR * kspad * RP * FRAG;
     const char *next_gemm = last ? nullptr : first_ptr(gi + 1);
     auto chunk_g0 = [&](int c) {  // first arena group of chunk c of X_gi
       const int g = xg0 + 2 * PF * c;
@@ -670,21 +681,21 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     };
     // INT8: the accumulators start at the bias 128 * sum_k X[k][n] of their column,
     // so the MFMAs accumulate u8(a).b directly.
-    int bias0[TNB];
+    int bias0[TNW];
 #pragma unroll
-    for (int j = 0; j < TNB; ++j)
-      bias0[j] = (KIND == K_I8) ? 128 * colsum[(gi % 3) * TN + j * 32 + (fresh_tid() & 31)] : 0;
+    for (int j = 0; j < TNW; ++j)
+      bias0[j] = (KIND == K_I8) ? 128 * colsum[(gi % 3) * TN + (cb0 + j) * 32 + (fresh_tid() & 31)] : 0;
     int *const csum_next = colsum + ((gi + 1) % 3) * TN;
     if (KIND == K_I8)
       for (int t = fresh_tid(); t < TN; t += NTH) colsum[((gi + 2) % 3) * TN + t] = 0;
 
-    Acc acc[RP][TNB];
+    Acc acc[RP][TNW];
     long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
     for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += PASS) {
 #pragma unroll
       for (int i = 0; i < RP; ++i)
 #pragma unroll
-        for (int j = 0; j < TNB; ++j)
+        for (int j = 0; j < TNW; ++j)
 #pragma unroll
           for (int r = 0; r < 16; ++r) acc[i][j][r] = bias0[j];
       const bool final_pass = rb0 + PASS >= Mp;
@@ -696,7 +707,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       // B fragments one step ahead, double buffered (PF is even, so the buffer of
       // step ks is ks & 1 across chunks); steps past the data (the last chunk's
       // padding) are skipped with a uniform branch.
-      BFrag<KIND, TNB> bq[2];
+      BFrag<KIND, TNW> bq[2];
       auto run_chunk = [&](int c, const AFrag<KIND>(&a)[PF][RP]) {
         const int g0 = chunk_g0(c), gn = chunk_g0(c + 1 < nchunks ? c + 1 : 0);
 #pragma unroll
@@ -705,7 +716,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
           if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
           else read_b(gn, 0, bq[0]);
           __builtin_amdgcn_sched_barrier(0);
-          mma_k<KIND, TNB>(bq[s & 1], a[s], acc);
+          mma_k<KIND, TNW>(bq[s & 1], a[s], acc);
           __builtin_amdgcn_sched_barrier(0);
         }
       };
@@ -725,7 +736,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
             if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
             else read_b(gn, 0, bq[0]);
             __builtin_amdgcn_sched_barrier(0);
-            mma_k<KIND, TNB>(bq[s & 1], a[s], acc);
+            mma_k<KIND, TNW>(bq[s & 1], a[s], acc);
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -752,7 +763,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       }
       apk = after;
       if (final_pass && !last && P.hold) held_rb0 = rb0;
-      else store_rows_impl<KIND, TNB, EPX>(P, gi, rb0, n0, arena, acc, csum_next);
+      else store_rows_impl<KIND, TN, TNW, EPX>(P, gi, rb0, n0, arena, acc, csum_next, cb0);
     }
     if ((long)wave * RP * 32 >= Mp && next_gemm) {  // no rows here: fetch the next GEMM's first chunk
       apk = next_gemm;
@@ -767,7 +778,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       continue;
     }
     __syncthreads();  // every wave is done reading X_gi
-    if (held_rb0 >= 0) store_rows_impl<KIND, TNB, EPX>(P, gi, held_rb0, n0, arena, acc, csum_next);
+    if (held_rb0 >= 0) store_rows_impl<KIND, TN, TNW, EPX>(P, gi, held_rb0, n0, arena, acc, csum_next, cb0);
     zero_pad(gi);
     __syncthreads();  // Y_gi complete
   }
@@ -782,10 +793,10 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <int KIND, int TNB, bool IMG, bool EPX>
+template <int KIND, int TNB, bool IMG, bool EPX, int G>
 __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   extern __shared__ uint4 arena[];
-  chain_tile<KIND, TNB, IMG, EPX>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
+  chain_tile<KIND, TNB, IMG, EPX, G>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
 }
 
 constexpr int RP_of(int kind) {
@@ -810,6 +821,13 @@ size_t arena_bytes(int kind, int S, int tile_n) {
   return groups * tile_n * 16 * (kind == K_F32X3 ? 2 : 1) + (kind == K_I8 ? 3 * tile_n * sizeof(int) : 0);
 }
 
+// Column groups of a fused tile (chain_tile's G): 256-column tiles, and INT8/BF16
+// tiles of 128 columns, give each wave one 32-column block and all rows; narrower
+// tiles give each wave all columns of a row band.
+constexpr int wave_groups(int kind, int tile_n) {
+  return tile_n == 256 ? 8 : ((tile_n == 128 && (kind == K_I8 || kind == K_BF16)) ? 4 : 1);
+}
+
 // Circular arena of S rows: X_0 (the B tile) at row 0, Y_i right after X_i (mod S)
 // and X_{i+1} = Y_i; every region spans its k-extent rounded up to a chunk and starts
 // on a chunk boundary (S is a multiple of a chunk). When max_i (Kc_i + Mc_i) rows fit
@@ -822,7 +840,7 @@ int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *x
   const long cs = CS_of(kind);
   auto rc = [cs](long v) { return (v + cs - 1) / cs * cs; };
   auto r32 = [](long v) { return (v + 31) / 32 * 32; };
-  const long pass = (long)NW * RP_of(kind) * 32;
+  const long pass = (long)(NW / wave_groups(kind, tile_n)) * RP_of(kind) * 32;  // rows per pass
   long S = rc(K[0]), wide = rc(K[0]);
   for (int i = 0; i < n; ++i) {
     const long kc = rc(K[i]);
@@ -846,7 +864,7 @@ int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *x
 }
 
 
-template <int KIND, int TNB>
+template <int KIND, int TNB, int G>
 hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   ChainParams P{};
   make_chain_params(c, &P);
@@ -854,8 +872,9 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   constexpr int TN = TNB * 32;
   const size_t lds = c.epi ? (size_t)P.epi_off + chain_epi_lds_bytes(c.M, c.num_gemm) : arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
-  auto kfn = c.img ? (c.epi ? chain_kernel<KIND, TNB, true, true> : chain_kernel<KIND, TNB, true, false>)
-                   : (c.epi ? chain_kernel<KIND, TNB, false, true> : chain_kernel<KIND, TNB, false, false>);
+  static_assert(wave_groups(KIND, TNB * 32) == G, "tile width and column groups disagree");
+  auto kfn = c.img ? (c.epi ? chain_kernel<KIND, TNB, true, true, G> : chain_kernel<KIND, TNB, true, false, G>)
+                   : (c.epi ? chain_kernel<KIND, TNB, false, true, G> : chain_kernel<KIND, TNB, false, false, G>);
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((unsigned)((c.N + TN - 1) / TN));
@@ -889,21 +908,25 @@ __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restri
   }
 }
 
-// Widest tile per kind: BF16 and INT8 keep two row blocks per wave, so wider tiles
-// would not fit their accumulators and double-buffered fragments in 256 VGPRs.
-constexpr int max_tile_of(int kind) { return (kind == K_BF16 || kind == K_I8) ? 64 : 128; }
+// Tile widths per kind: 32, 64, 96 and 128 columns with every wave on all columns
+// (BF16 and INT8 stop at 64: two row blocks per wave leave no registers for more),
+// 128 (INT8, BF16) and 256 with the waves split over the columns (wave_groups).
+constexpr bool tile_ok_of(int kind, int tile_n) {
+  return tile_n == 32 || tile_n == 64 || tile_n == 128 || tile_n == 256 ||
+         (tile_n == 96 && kind != K_BF16 && kind != K_I8);
+}
 
 template <int KIND>
 hipError_t launch_chain_k(const ChainArgs &c, hipStream_t s) {
+  constexpr bool two_rows = KIND == K_BF16 || KIND == K_I8;
   switch (c.tile_n) {
-    case 32: return launch_chain_t<KIND, 1>(c, s);
-    case 64: return launch_chain_t<KIND, 2>(c, s);
+    case 32: return launch_chain_t<KIND, 1, 1>(c, s);
+    case 64: return launch_chain_t<KIND, 2, 1>(c, s);
+    case 128: return launch_chain_t<KIND, 4, two_rows ? 4 : 1>(c, s);
+    case 256: return launch_chain_t<KIND, 8, 8>(c, s);
   }
-  if constexpr (max_tile_of(KIND) == 128) {
-    switch (c.tile_n) {
-      case 96: return launch_chain_t<KIND, 3>(c, s);
-      case 128: return launch_chain_t<KIND, 4>(c, s);
-    }
+  if constexpr (!two_rows) {
+    if (c.tile_n == 96) return launch_chain_t<KIND, 3, 1>(c, s);
   }
   return hipErrorInvalidValue;
 }
@@ -938,7 +961,11 @@ size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gem
 
 int chain_fused_max_gemms() { return MAXG; }
 
-int chain_fused_max_tile(int kind) { return max_tile_of(kind); }
+bool chain_fused_tile_ok(int kind, int tile_n) { return tile_ok_of(kind, tile_n); }
+
+int chain_fused_rows_per_wave(int kind) { return RP_of(kind) * 32; }
+
+int chain_fused_wave_groups(int kind, int tile_n) { return wave_groups(kind, tile_n); }
 
 size_t packed_a_bytes(int kind, long M, long K) {
   switch (kind) {

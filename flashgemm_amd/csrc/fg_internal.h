@@ -280,7 +280,12 @@ hipError_t launch_chain_fused(const ChainArgs &c, hipStream_t s);
 size_t chain_fused_lds_bytes(int kind, const long *M, const long *K, int num_gemm, int tile_n, bool epi = false);
 size_t chain_epi_lds_bytes(const long *M, int num_gemm);
 int chain_fused_max_gemms();
-int chain_fused_max_tile(int kind);  // widest fused tile_n for the kind
+// Fused tile widths: 32, 64, 128, 256 (96 for the FP32 kinds). Tiles of 256 (and 128
+// for INT8/BF16) split the 8 waves over 32-column blocks (chain_fused_wave_groups
+// groups), each walking rows in passes of (8 / groups) * chain_fused_rows_per_wave.
+bool chain_fused_tile_ok(int kind, int tile_n);
+int chain_fused_wave_groups(int kind, int tile_n);
+int chain_fused_rows_per_wave(int kind);
 constexpr size_t LDS_BYTES_MAX = 160 * 1024;
 
 // Grouped persistent launch. The device table is a GroupTable header followed by

@@ -82,7 +82,7 @@ typedef struct fg_chain_opts {
   int ref_threads; /* thread count the REFERENCE layout follows (>= 1)   */
   int f32_mode;    /* FG_F32_MFMA (default) or FG_F32_BF16X3            */
   int fuse;        /* FG_FUSE_AUTO (default), FG_FUSE_OFF, FG_FUSE_ON    */
-  int tile_n;      /* fused tile width: 0 = planner, or 32, 64, 96, 128  */
+  int tile_n;      /* fused tile width: 0 = planner, or 32, 64, 96, 128, 256 (96 not for INT8/BF16) */
   int splits;      /* split-K of unfused GEMMs: 0 = planner              */
   int a_packed;    /* 1: A[i] come from fg_pack_weight (fused path only) */
 } fg_chain_opts;

@@ -228,11 +228,11 @@ def test_int8_chain_bit_exact_math_layout(gpu, K1, N, Ms, fuse):
     assert np.array_equal(got, want), f"{(got != want).sum()} of {got.size} differ"
 
 
-@pytest.mark.parametrize("tile_n", [32, 64, 96, 128])
+@pytest.mark.parametrize("tile_n", [32, 64, 96, 128, 256])
 def test_int8_chain_every_tile_width(gpu, tile_n):
     rng = np.random.default_rng(tile_n)
     B, As = rand_chain(rng, fg.DTYPE_INT8, 128, 300, [64, 96, 48], full_int8=True)
-    if tile_n > 64:  # INT8 fused tiles are at most 64 columns wide
+    if tile_n == 96:  # INT8 tiles: 32 and 64 columns on every wave, 128 and 256 split over the waves
         with pytest.raises(fg.FlashgemmError) as e:
             run_chain(fg.DTYPE_INT8, B, As, fuse=fg.FUSE_ON, tile_n=tile_n)
         assert e.value.status == fg.ERR_UNSUPPORTED
@@ -242,10 +242,17 @@ def test_int8_chain_every_tile_width(gpu, tile_n):
 
 
 @pytest.mark.parametrize("f32_mode", [fg.F32_MFMA, fg.F32_BF16X3])
-@pytest.mark.parametrize("tile_n", [32, 64, 96, 128])
+@pytest.mark.parametrize("tile_n", [32, 64, 96, 128, 256])
 def test_f32_chain_every_tile_width(gpu, tile_n, f32_mode):
     rng = np.random.default_rng(tile_n + 7)
     B, As = rand_chain(rng, fg.DTYPE_F32, 100, 300, [72, 40, 136])
+    if not fg.chain_plan(fg.DTYPE_F32, 300, [72, 40, 136], [100, 72, 40],
+                         fg.opts(tile_n=tile_n, f32_mode=f32_mode))[0]:
+        # 256 fp32 columns take 4 KB of LDS per row: this chain's arena does not fit
+        with pytest.raises(fg.FlashgemmError) as e:
+            run_chain(fg.DTYPE_F32, B, As, fuse=fg.FUSE_ON, tile_n=tile_n, f32_mode=f32_mode)
+        assert e.value.status == fg.ERR_UNSUPPORTED
+        return
     got = run_chain(fg.DTYPE_F32, B, As, fuse=fg.FUSE_ON, tile_n=tile_n, f32_mode=f32_mode)
     tol = TOL_F32 if f32_mode == fg.F32_MFMA else TOL_X3
     assert O.rel_fro(got, O.chain_f32(B, As)) <= tol

@@ -545,6 +545,8 @@ static int conv_geom(const fg_conv_geom *g, ImgGeom *ig, long *K1, long *N) {
     ig->dw = (int)g->dilation_w;
     ig->OH = (int)oh;
     ig->OW = (int)ow;
+    ig->vec4 = 0;
+    ig->nq = 0;
   }
   if (K1) *K1 = g->channels * g->kernel_h * g->kernel_w;
   if (N) *N = g->batch * oh * ow;
@@ -572,6 +574,11 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
   long K1, N;
   if ((rc = conv_geom(g, &ig, &K1, &N))) return rc;
   if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
+  {
+    const long elems = g->batch * g->channels * g->height * g->width;
+    ig.nq = (int)(elems / 4);
+    ig.vec4 = dtype == FG_DTYPE_INT8 && ig.sw == 1 && ig.OW % 4 == 0 && elems % 4 == 0 && ((uintptr_t)X & 3) == 0;
+  }
   if (K[0] != K1) return set_error(FG_ERR_INVALID, "conv chain: K[0]=%ld but the patches have K1=%ld rows", K[0], K1);
   if (!X || !C) return set_error(FG_ERR_INVALID, "conv chain: X or C is NULL");
   EpiParams p{};

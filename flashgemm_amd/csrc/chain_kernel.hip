@@ -256,6 +256,11 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
       }
     }
     const int k0 = g * T::KG, kk = img.kh * img.kw;
+    if (k0 >= K1) {  // a group of the chunk padding (wave-uniform: a wave covers one group)
+#pragma unroll
+      for (int r = 0; r < T::KG; ++r) x[r] = Raw{};
+      return;
+    }
     int c = k0 / kk, rr = k0 - c * kk;
     int ky = rr / img.kw, kx = rr - ky * img.kw;
 #pragma unroll
@@ -263,6 +268,34 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
       const bool rok = k0 + r < K1;
       const int dy = ky * img.dh, dx = kx * img.dw;
       const int roff = c * HW + dy * img.W + dx;
+      if constexpr (T::ESZ == 1) {
+        if (img.vec4) {
+          // INT8 with stride 1 along x and OW % 4 == 0: the 4 columns are 4 adjacent
+          // pixels of one image row, bytes base .. base + 3. Two aligned dword loads
+          // (indices clamped into the image; a clamped dword only feeds bytes that
+          // lie outside it, which the mask clears) and a byte funnel shift.
+          const int base = cbase[0] + roff, ix0 = cx[0] + dx;
+          const bool rowok = rok && (unsigned)(cy[0] + dy) < (unsigned)img.H;
+          const int q = (base - (base & 3)) >> 2;
+          const int ql = q < 0 ? 0 : (q >= img.nq ? img.nq - 1 : q);
+          const int qh = q + 1 < 0 ? 0 : (q + 1 >= img.nq ? img.nq - 1 : q + 1);
+          const uint32_t lo = ((const uint32_t *)B)[ql], hi = ((const uint32_t *)B)[qh];
+          const uint32_t v = __builtin_amdgcn_alignbyte(hi, lo, (uint32_t)(base & 3));
+          uint32_t mask = 0u;
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (rowok && cok[j] && (unsigned)(ix0 + j) < (unsigned)img.W) mask |= 0xffu << (8 * j);
+          x[r] = v & mask;
+          if (++kx == img.kw) {
+            kx = 0;
+            if (++ky == img.kh) {
+              ky = 0;
+              ++c;
+            }
+          }
+          continue;
+        }
+      }
       uint32_t e[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {

@@ -215,6 +215,8 @@ struct ImgGeom {
   int C, H, W;
   int kh, kw, sh, sw, ph, pw, dh, dw;
   int OH, OW;
+  int vec4;  // INT8: sw == 1, OW % 4 == 0, the image 4-byte aligned and a whole number of dwords
+  int nq;    // image size in dwords (vec4)
 };
 // Element size `esz` (1, 2 or 4 bytes): B (K1 x N, K1 = C*kh*kw, N = batch*OH*OW).
 hipError_t launch_im2col(int esz, const void *X, void *B, const ImgGeom &g, long K1, long N, hipStream_t s);

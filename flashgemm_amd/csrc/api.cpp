@@ -106,10 +106,13 @@ int plan_splits(long M, long N, long K, int kind) {
 }
 
 // Estimated cycles of a fused plan on one CU. Tiles run in rounds of one per CU. A
-// tile costs about 10^4 cycles of staging, barriers and epilogues plus the MFMA
-// issue of its busiest wave: passes_i x k-steps_i x (row blocks x column blocks per
-// wave) MFMA groups per GEMM, at the MFMA's cycles per group times the two waves
-// that share a SIMD (INT8/BF16: measured ~1.5x the MFMA rate with the weight stream).
+// tile costs about 10^4 cycles of staging, barriers and epilogues plus, per GEMM, the
+// larger of two bounds: the busiest wave's MFMA issue (passes x k-steps x its row x
+// column blocks, at the MFMA's cycles per block times the two waves that share a
+// SIMD; INT8/BF16 measured ~1.5x the MFMA rate beside the weight stream) and the
+// weight stream itself: ~52 bytes per cycle per CU from L2 (measured for the pattern)
+// with all waves on all columns; with the waves split over G column groups every
+// group reads all of A_i, and measured conv-shaped chains fit half that rate.
 static double fused_cost(int kind, long N, int num_gemm, const long *M, const long *K, int tn) {
   const int groups = chain_fused_wave_groups(kind, tn);
   const int rows = chain_fused_rows_per_wave(kind);
@@ -117,12 +120,17 @@ static double fused_cost(int kind, long N, int num_gemm, const long *M, const lo
   const int blocks = (rows / 32) * (tn / 32 / groups);
   const long kc = kind == K_F32 ? 8 : (kind == K_I8 ? 32 : 16);
   const double unit = kind == K_F32 ? 4 * 64 * 2 : (kind == K_F32X3 ? 3 * 32 * 2 : 32 * 2 * 1.5);
-  double t = 0;
-  for (int i = 0; i < num_gemm; ++i)
-    t += (double)((M[i] + pass - 1) / pass) * ((K[i] + kc - 1) / kc) * blocks * unit;
+  const double esz = kind == K_I8 ? 1 : (kind == K_BF16 ? 2 : 4);  // packed bytes per weight element
+  double t = 1e4;
+  for (int i = 0; i < num_gemm; ++i) {
+    const double mfma = (double)((M[i] + pass - 1) / pass) * ((K[i] + kc - 1) / kc) * blocks * unit;
+    const double stream = (double)groups * ((M[i] + rows - 1) / rows * rows) * ((K[i] + kc - 1) / kc * kc) * esz /
+                          (groups > 1 ? 26.0 : 52.0);
+    t += mfma > stream ? mfma : stream;
+  }
   const long tiles = (N + tn - 1) / tn;
   const long rounds = (tiles + num_cus() - 1) / num_cus();
-  return (double)rounds * (t + 1e4);
+  return (double)rounds * t;
 }
 
 // Fused tile width: the cheapest tile (fused_cost) that fits LDS; within 3% the one

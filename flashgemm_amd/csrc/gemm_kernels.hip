@@ -333,13 +333,18 @@ __device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int k
 
 // Map a linear workgroup id to (tile_m, tile_n). Consecutive ids are dispatched
 // round-robin over the 8 XCDs; regroup them so that each XCD receives a contiguous
-// run of logical tiles (sharing B column panels in its own L2).
+// run of logical tiles, and walk the tiles in bands of 4 tile rows (all columns of a
+// band, 4 rows at a time), so the tiles an XCD runs together share few A row panels
+// and few B column panels in its L2 (2048^3: 4 + 8 panels instead of 16 + 2).
 __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &tm, int &tn) {
-  int total = tiles_m * tiles_n;
+  constexpr int BAND = 4;
+  const int total = tiles_m * tiles_n;
   int L = bid;
   if ((total & 7) == 0) L = (bid & 7) * (total >> 3) + (bid >> 3);
-  tm = L % tiles_m;
-  tn = L / tiles_m;
+  const int band = BAND * tiles_n, st = L / band, l = L - st * band;
+  const int rows = tiles_m - st * BAND < BAND ? tiles_m - st * BAND : BAND;  // the last band may be short
+  tm = st * BAND + l % rows;
+  tn = l / rows;
 }
 
 // TRANS bit 0: A is stored transposed (K x M, op(A) = A^T); bit 1: B is stored

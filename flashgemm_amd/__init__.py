@@ -12,8 +12,9 @@ Two layers, mirroring include/flashgemm.h:
   arrays: ``flashgemm_set_thread_num``, ``flashgemm_multi_f32f32f32``,
   ``flashgemm_multi_bf16bf16f32``, ``flashgemm_multi_int8uint8int32``,
   ``LibShalom_set_thread_nums``, ``LibShalom_sgemm_mp``;
-* the device API (``chain``, ``sgemm``, ``f64_chain``, ``error_norms`` ...) on
-  device buffers given as torch tensors or raw addresses, enqueued on a stream.
+* the device API (``chain``, ``chain_epi``, ``chain_conv``, ``im2col``, ``sgemm``,
+  ``sgemm_t``, ``f64_chain``, ``error_norms`` ...) on device buffers given as torch
+  tensors or raw addresses, enqueued on a stream.
 """
 from __future__ import annotations
 

@@ -136,23 +136,28 @@ def test_fused_conv_chain_int8_tiles_tails(gpu, tile_n):
     assert np.array_equal(run_conv(fg.DTYPE_INT8, X, g, As, fuse=fg.FUSE_ON, tile_n=tile_n), want)
 
 
-def test_fused_conv_chain_int8_unaligned_image(gpu):
-    # an image at an odd byte address takes the element-wise gather instead of the
-    # dword one: same bits
+@pytest.mark.parametrize("dtype", [fg.DTYPE_INT8, fg.DTYPE_BF16])
+def test_fused_conv_chain_unaligned_image(gpu, dtype):
+    # an image that is not dword aligned takes the element-wise gather instead of the
+    # dword one: same bits as the aligned image
     import torch
     g = fg.conv_geom(1, 4, 16, 16, 3, 3, pad=1)
     rng = np.random.default_rng(9)
-    X = image(rng, fg.DTYPE_INT8, (1, 4, 16, 16))
+    X = image(rng, dtype, (1, 4, 16, 16))
     K1, N = fg.conv_dims(g)
-    As = [rng.integers(-127, 128, (m, k)).astype(np.int8) for m, k in ((32, K1), (16, 32))]
-    raw = torch.zeros(X.size + 1, dtype=torch.int8, device="cuda")
-    raw[1:] = torch.from_numpy(X.reshape(-1)).cuda()
-    C = empty((16, N), torch.int32)
-    fg.chain_conv(fg.DTYPE_INT8, C, raw.data_ptr() + 1, g, [dev(a) for a in As], o=fg.opts(fuse=fg.FUSE_ON))
+    As = weights(rng, dtype, [32, 16], K1)
+    tdt = torch.int8 if dtype == fg.DTYPE_INT8 else torch.int16
+    flat = dev(X.reshape(-1))
+    raw = torch.zeros(flat.numel() + 1, dtype=tdt, device="cuda")
+    raw[1:] = flat  # one element in: 1 byte (INT8) or 2 bytes (BF16) past a dword
+    C = empty((16, N), out_dtype(dtype))
+    fg.chain_conv(dtype, C, raw.data_ptr() + raw.element_size(), g, [dev(a) for a in As],
+                  o=fg.opts(fuse=fg.FUSE_ON))
     torch.cuda.synchronize()
-    want = O.chain_int8(O.im2col(X, 3, 3, (1, 1), (1, 1), (1, 1)), As, O.LAYOUT_MATH)
-    assert np.array_equal(C.cpu().numpy(), want)
-    assert np.array_equal(run_conv(fg.DTYPE_INT8, X, g, As, fuse=fg.FUSE_ON), want)
+    aligned = run_conv(dtype, X, g, As, fuse=fg.FUSE_ON)
+    assert np.array_equal(C.cpu().numpy(), aligned)
+    if dtype == fg.DTYPE_INT8:
+        assert np.array_equal(aligned, O.chain_int8(O.im2col(X, 3, 3, (1, 1), (1, 1), (1, 1)), As, O.LAYOUT_MATH))
 
 
 @pytest.mark.parametrize("dtype", [fg.DTYPE_F32, fg.DTYPE_INT8])

@@ -25,6 +25,10 @@
 //     taken while the tile is staged and those of each Y_i by the epilogue that
 //     writes it, and they seed the next GEMM's accumulators;
 //   * only the last GEMM writes HBM (C, fp32 or int32, row-major).
+// Variants (template parameters of chain_kernel): wide tiles whose waves split over
+// the columns (G, for chains with few rows per GEMM), B gathered from an NCHW image
+// while it is staged (IMG, the convolution front end), and epilogues with per-row
+// bias / scale, INT8 zero point and ReLU (EPX).
 #include "fg_internal.h"
 
 namespace fg {

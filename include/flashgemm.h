@@ -209,7 +209,7 @@ typedef struct fg_chain_req {
 } fg_chain_req;
 
 /* All requests in one persistent launch: each request's columns are cut into tiles
- * (32 wide; 64 for INT8) that run the whole chain on chip. Requests whose
+ * (32 columns wide) that run the whole chain on chip. Requests whose
  * intermediates do not fit on chip at that width run unfused on the same stream
  * after it. Asynchronous. */
 int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream);

@@ -118,6 +118,8 @@ def lib() -> ctypes.CDLL:
                                     ctypes.POINTER(Epilogue), ctypes.POINTER(ChainOpts), vp]
         L.fg_chain_epi.argtypes = [i, vp, vp, l, i, vpp, lp, lp, ctypes.POINTER(Epilogue),
                                    ctypes.POINTER(ChainOpts), vp]
+        L.fg_chain_conv_host.argtypes = [i, vp, vp, ctypes.POINTER(ConvGeom), i, vpp, lp, lp,
+                                         ctypes.POINTER(ChainOpts)]
         L.fg_sgemm.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_t.argtypes = [i, i, vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_host.argtypes = [vp, vp, vp, l, l, l, i]
@@ -250,6 +252,13 @@ def chain_conv(dtype: int, C, X, g: ConvGeom, As, Ms=None, Ks=None, o: ChainOpts
     _check(lib().fg_chain_conv(dtype, _addr(C), _addr(X), ctypes.byref(g), n, ptrs, M, K,
                                ctypes.byref(epi) if epi is not None else None,
                                ctypes.byref(o) if o is not None else None, _stream(stream)))
+
+
+def chain_conv_host(dtype: int, C, X, g: ConvGeom, As, o: ChainOpts | None = None) -> None:
+    """Blocking convolution chain on host numpy arrays (X is NCHW)."""
+    n, ptrs, M, K = _arrays(As)
+    _check(lib().fg_chain_conv_host(dtype, _addr(C), _addr(X), ctypes.byref(g), n, ptrs, M, K,
+                                    ctypes.byref(o) if o is not None else None))
 
 
 def chain_epi(dtype: int, C, B, As, epi: Epilogue | None, N: int | None = None, Ms=None, Ks=None,

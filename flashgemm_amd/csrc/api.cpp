@@ -614,6 +614,10 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
 }
 
 // -------------------------------------------------------- host-pointer forms --
+// The host forms share one stream and its scratch ("host_*" buffers): one mutex
+// serializes them.
+static std::mutex g_host_mu;
+
 static hipStream_t host_stream() {
   static hipStream_t s = nullptr;
   static std::once_flag once;
@@ -627,8 +631,7 @@ int fg_chain_host(int dtype, void *C, const void *B, long N, int num_gemm, const
   if ((rc = ensure_init())) return rc;
   if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
   if (!B || !C) return set_error(FG_ERR_INVALID, "B or C is NULL");
-  static std::mutex host_mu;  // the host stream's scratch is shared by host calls
-  std::lock_guard<std::mutex> lk(host_mu);
+  std::lock_guard<std::mutex> lk(g_host_mu);
   hipStream_t s = host_stream();
   size_t e = dtype_bytes(dtype);
   void *dB, *dC, *dA[8];
@@ -643,6 +646,34 @@ int fg_chain_host(int dtype, void *C, const void *B, long N, int num_gemm, const
   }
   if ((rc = hip_status(hipMemcpyAsync(dB, B, (size_t)K[0] * N * e, hipMemcpyHostToDevice, s), "H2D B"))) return rc;
   if ((rc = fg_chain(dtype, dC, dB, N, num_gemm, (const void *const *)dA, M, K, opts, s))) return rc;
+  if ((rc = hip_status(hipMemcpyAsync(C, dC, (size_t)M[num_gemm - 1] * N * 4, hipMemcpyDeviceToHost, s), "D2H C")))
+    return rc;
+  return hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+int fg_chain_conv_host(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
+                       const long *M, const long *K, const fg_chain_opts *opts) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  long K1, N;
+  if ((rc = conv_geom(g, nullptr, &K1, &N))) return rc;
+  if ((rc = validate_chain(dtype, N, num_gemm, A, M, K))) return rc;
+  if (!X || !C) return set_error(FG_ERR_INVALID, "conv chain: X or C is NULL");
+  std::lock_guard<std::mutex> lk(g_host_mu);
+  hipStream_t s = host_stream();
+  const size_t e = dtype_bytes(dtype), xbytes = (size_t)g->batch * g->channels * g->height * g->width * e;
+  void *dX, *dC, *dA[8];
+  if ((rc = scratch(s, "host_X", xbytes, &dX))) return rc;
+  if ((rc = scratch(s, "host_C", (size_t)M[num_gemm - 1] * N * 4, &dC))) return rc;
+  for (int i = 0; i < num_gemm; ++i) {
+    char name[16];
+    snprintf(name, sizeof name, "host_A%d", i);
+    if ((rc = scratch(s, name, (size_t)M[i] * K[i] * e, &dA[i]))) return rc;
+    if ((rc = hip_status(hipMemcpyAsync(dA[i], A[i], (size_t)M[i] * K[i] * e, hipMemcpyHostToDevice, s), "H2D A")))
+      return rc;
+  }
+  if ((rc = hip_status(hipMemcpyAsync(dX, X, xbytes, hipMemcpyHostToDevice, s), "H2D X"))) return rc;
+  if ((rc = fg_chain_conv(dtype, dC, dX, g, num_gemm, (const void *const *)dA, M, K, nullptr, opts, s))) return rc;
   if ((rc = hip_status(hipMemcpyAsync(C, dC, (size_t)M[num_gemm - 1] * N * 4, hipMemcpyDeviceToHost, s), "D2H C")))
     return rc;
   return hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");

@@ -192,6 +192,9 @@ int fg_im2col(int dtype, void *B, const void *X, const fg_conv_geom *g, void *st
  * formed in library scratch first. */
 int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
                   const long *M, const long *K, const fg_epilogue *epi, const fg_chain_opts *opts, void *stream);
+/* Same with host pointers (no epilogue extensions); blocking, like fg_chain_host. */
+int fg_chain_conv_host(int dtype, void *C, const void *X, const fg_conv_geom *g, int num_gemm, const void *const *A,
+                       const long *M, const long *K, const fg_chain_opts *opts);
 
 /* ---- grouped launches (no reference counterpart: the reference runs one chain
  * per call). Many chains of uneven shapes and mixed dtypes, mathematical layout,

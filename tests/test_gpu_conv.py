@@ -188,6 +188,22 @@ def test_reference_layout_conv_chain(gpu):
     assert np.array_equal(got, O.chain_int8(B, As, O.LAYOUT_REFERENCE, 4))
 
 
+@pytest.mark.parametrize("dtype", [fg.DTYPE_INT8, fg.DTYPE_BF16])
+def test_conv_chain_host_form(gpu, dtype):
+    # fg_chain_conv_host (host arrays, blocking) gives the device call's bits
+    t = GEOMS["5x5_s2_b2"]
+    g = geom_of(t)
+    rng = np.random.default_rng(10)
+    X = image(rng, dtype, t[:4])
+    K1, N = fg.conv_dims(g)
+    As = weights(rng, dtype, [40, 24], K1)
+    C = np.empty((24, N), np.int32 if dtype == fg.DTYPE_INT8 else np.float32)
+    fg.chain_conv_host(dtype, C, np.ascontiguousarray(X), g, [np.ascontiguousarray(a) for a in As])
+    assert np.array_equal(C, run_conv(dtype, X, g, As))
+    if dtype == fg.DTYPE_INT8:
+        assert np.array_equal(C, O.chain_int8(O.im2col(X, t[4], t[5], t[6], t[7], t[8]), As, O.LAYOUT_MATH))
+
+
 def test_conv_chain_rejects_wrong_k1(gpu):
     g = fg.conv_geom(1, 4, 8, 8, 3, 3, pad=1)  # K1 = 36
     rng = np.random.default_rng(8)

@@ -170,9 +170,12 @@ template <int KIND, int TNB> struct BFrag {
   uint4 lo[KIND == K_F32X3 ? TNB : 1];
 };
 
+// FP32 alternates its four MFMAs of a step between acc and acc2 (summed when the
+// pass ends), so consecutive MFMAs of a wave never wait on each other's result.
 template <int KIND, int TNB>
 __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KIND> (&af)[CT<KIND>::RP],
-                                      typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
+                                      typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB],
+                                      typename CT<KIND>::Acc (&acc2)[CT<KIND>::RP][TNB]) {
   constexpr int RP = CT<KIND>::RP;
 #pragma unroll
   for (int j = 0; j < TNB; ++j) {
@@ -190,8 +193,10 @@ __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KI
         const f32x4 av = __builtin_bit_cast(f32x4, af[i].v[0]);
         const f32x4 bv = __builtin_bit_cast(f32x4, bf.b[j]);
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
+        for (int s = 0; s < 4; s += 2) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s], bv[s], acc[i][j], 0, 0, 0);
+          acc2[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s + 1], bv[s + 1], acc2[i][j], 0, 0, 0);
+        }
       } else if (KIND == K_BF16) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, af[i].v[0]),
                                                             __builtin_bit_cast(bf16x8, bf.b[j]), acc[i][j], 0, 0, 0);
@@ -485,11 +490,11 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
             if (KIND == K_I8) {
               int v = (int)acc[i][j][r];
               if (EPX) v = epi_i32(v, (int)lb[m], ex.relu);
-              *(int *)(base + ro * rs) = v;
+              __builtin_nontemporal_store(v, (int *)(base + ro * rs));
             } else {
               float v = (float)acc[i][j][r];
               if (EPX) v = epi_f(v, has_scale, ls[m], has_bias, __builtin_bit_cast(float, lb[m]), ex.relu);
-              *(float *)(base + ro * rs) = v;
+              __builtin_nontemporal_store(v, (float *)(base + ro * rs));
             }
           }
         }
@@ -727,6 +732,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       for (int t = fresh_tid(); t < TN; t += NTH) colsum[((gi + 2) % 3) * TN + t] = 0;
 
     Acc acc[RP][TNW];
+    Acc acc2[RP][TNW];  // FP32: the odd MFMAs of every step (mma_k)
     long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
     for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += PASS) {
 #pragma unroll
@@ -734,7 +740,10 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
 #pragma unroll
         for (int j = 0; j < TNW; ++j)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) acc[i][j][r] = bias0[j];
+          for (int r = 0; r < 16; ++r) {
+            acc[i][j][r] = bias0[j];
+            acc2[i][j][r] = 0;
+          }
       const bool final_pass = rb0 + PASS >= Mp;
       const char *nxt = final_pass ? next_gemm : apk + pass_stride;
       // With no next pass the prefetch re-reads this pass's first chunk: the loads
@@ -753,7 +762,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
           if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
           else read_b(gn, 0, bq[0]);
           __builtin_amdgcn_sched_barrier(0);
-          mma_k<KIND, TNW>(bq[s & 1], a[s], acc);
+          mma_k<KIND, TNW>(bq[s & 1], a[s], acc, acc2);
           __builtin_amdgcn_sched_barrier(0);
         }
       };
@@ -773,7 +782,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
             if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
             else read_b(gn, 0, bq[0]);
             __builtin_amdgcn_sched_barrier(0);
-            mma_k<KIND, TNW>(bq[s & 1], a[s], acc);
+            mma_k<KIND, TNW>(bq[s & 1], a[s], acc, acc2);
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -799,6 +808,14 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
         load_chunk(after, fa[0]);
       }
       apk = after;
+      if (KIND == K_F32) {
+#pragma unroll
+        for (int i = 0; i < RP; ++i)
+#pragma unroll
+          for (int j = 0; j < TNW; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] += acc2[i][j][r];
+      }
       if (final_pass && !last && P.hold) held_rb0 = rb0;
       else store_rows_impl<KIND, TN, TNW, EPX>(P, gi, rb0, n0, arena, acc, csum_next, cb0);
     }

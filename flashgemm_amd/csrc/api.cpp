@@ -362,10 +362,13 @@ static ChainArgs chain_args(int kind, const void *B, void *C, long N, int num_ge
 }
 
 // Fused launch: unless the weights come packed, pack every A_i into MFMA fragment
-// order first (per-stream scratch, one launch for the whole chain).
+// order first (per-stream scratch, one launch for the whole chain). FP32 weights that
+// need no padding are read as they are (chain_raw_weights_ok): no pack launch.
 static int run_fused(ChainArgs c, int a_packed, hipStream_t s) {
   int rc;
-  if (!a_packed) {
+  if (!a_packed && chain_raw_weights_ok(c)) {
+    c.raw_a = 1;
+  } else if (!a_packed) {
     size_t off[CHAIN_MAXG], total = 0;
     for (int i = 0; i < c.num_gemm; ++i) {
       off[i] = total;

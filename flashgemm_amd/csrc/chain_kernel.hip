@@ -599,7 +599,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 // NW / G waves of its group. Wide tiles with G = TNB (one 32-column block per wave)
 // keep every wave busy in chains whose M_i are a few row blocks, where G = 1 would
 // leave most waves without rows.
-template <int KIND, int TNB, bool IMG = false, bool EPX = false, int G = 1>
+template <int KIND, int TNB, bool IMG = false, bool EPX = false, int G = 1, bool RAWA = false>
 __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
@@ -629,26 +629,39 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   // fragment (ks, i) of a pass at base + (ks * RP + i) * FRAG. The chunk pointers are
   // wave-uniform (scalar registers); each load adds the lane's 16-byte offset, so
   // the weight stream costs one vector register for addressing.
+  //
+  // RAW (FP32 with every M_i % 32 == 0 and K_i a whole number of chunks): P.A[gi] are
+  // the row-major weights themselves, read in the packed fragments' element order
+  // (lane h * 32 + r: row r of the block, k 4h..4h+3 of the step) at row stride K_gi,
+  // so the per-call pack is skipped; the lane offset then depends on the GEMM and
+  // travels with the chunk pointer (a chunk of the next GEMM is fetched early).
+  constexpr bool RAW = RAWA && KIND == K_F32;
+  static_assert(!RAW || RP == 1, "row-major weights: one row block per wave pass");
   auto first_ptr = [&](int gi) -> const char * {
     if ((long)wave * RP * 32 >= (P.M[gi] + 31) / 32 * 32) return nullptr;
+    if (RAW) return (const char *)P.A[gi] + (size_t)wave * RP * 32 * P.K[gi] * 4;
     return (const char *)P.A[gi] + (size_t)wave * kspad_of<KIND>(P.K[gi]) * RP * FRAG;
   };
   const unsigned lane16 = (unsigned)lane * 16u;
+  auto lane_of = [&](int gi) -> unsigned {
+    return RAW ? (unsigned)(((long)l32 * P.K[gi] + 4 * h) * 4) : lane16;
+  };
+  constexpr size_t STEP = RAW ? (size_t)T::KC * 4 : (size_t)RP * FRAG;  // bytes per k-step
   AFrag<KIND> fa[2][PF][RP];
   // the RP fragments of k-step s of the chunk at base
-  auto load_step = [&](const char *base, int s, AFrag<KIND>(&dst)[RP]) {
+  auto load_step = [&](const char *base, unsigned lo, int s, AFrag<KIND>(&dst)[RP]) {
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
-      const char *p = base + (size_t)(s * RP + i) * FRAG;
-      dst[i].v[0] = *(const uint4 *)(p + lane16);
-      if (KIND == K_F32X3) dst[i].v[1] = *(const uint4 *)(p + 1024 + lane16);
+      const char *p = base + (size_t)s * STEP + (size_t)i * FRAG;
+      dst[i].v[0] = *(const uint4 *)(p + lo);
+      if (KIND == K_F32X3) dst[i].v[1] = *(const uint4 *)(p + 1024 + lo);
     }
   };
-  auto load_chunk = [&](const char *base, AFrag<KIND>(&dst)[PF][RP]) {
+  auto load_chunk = [&](const char *base, unsigned lo, AFrag<KIND>(&dst)[PF][RP]) {
 #pragma unroll
-    for (int s = 0; s < PF; ++s) load_step(base, s, dst[s]);
+    for (int s = 0; s < PF; ++s) load_step(base, lo, s, dst[s]);
   };
-  constexpr size_t CHUNK = (size_t)PF * RP * FRAG;
+  constexpr size_t CHUNK = (size_t)PF * STEP;
 
   // B fragments of one k-step: column block j of this lane at slot
   // (g0 + 2 s + h) * TN + 32 j + l32 for step s of the chunk whose first group is g0
@@ -668,11 +681,12 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   }
   // invariant: at the start of each pass fa[0] holds (or has in flight) chunk 0
   const char *apk = first_ptr(0);
+  unsigned alane = lane_of(0);  // lane offset of the GEMM apk belongs to
   const long K1c = (P.K[0] + CS - 1) / CS * CS;
   // A wave with no rows in GEMM 0 loads (and ignores) wave 0's first chunk, which
   // always exists: the loads stay unconditional and the wait counts after them exact.
   stage_b<KIND, TN, IMG>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img,
-                         [&] { load_chunk(apk ? apk : (const char *)P.A[0], fa[0]); });
+                         [&] { load_chunk(apk ? apk : (const char *)P.A[0], alane, fa[0]); });
   if (EPX) {
     // every GEMM's per-row bias and scale into LDS (bias bits, then scales, rows of
     // GEMM g after those of GEMMs < g): the epilogues read them with LDS loads, so
@@ -715,8 +729,9 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     const int kspad = (int)kspad_of<KIND>(K);
     const int nchunks = kspad / PF;
     const int ksteps = (int)((K + T::KC - 1) / T::KC);  // k-steps with real data
-    const size_t pass_stride = (size_t)NWR * kspad * RP * FRAG;
+    const size_t pass_stride = RAW ? (size_t)NWR * RP * 32 * K * 4 : (size_t)NWR * kspad * RP * FRAG;
     const char *next_gemm = last ? nullptr : first_ptr(gi + 1);
+    const unsigned next_lane = last ? alane : lane_of(gi + 1);
     auto chunk_g0 = [&](int c) {  // first arena group of chunk c of X_gi
       const int g = xg0 + 2 * PF * c;
       return g < SG ? g : g - SG;
@@ -746,6 +761,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
           }
       const bool final_pass = rb0 + PASS >= Mp;
       const char *nxt = final_pass ? next_gemm : apk + pass_stride;
+      const unsigned after_lane = (final_pass && next_gemm) ? next_lane : alane;
       // With no next pass the prefetch re-reads this pass's first chunk: the loads
       // stay unconditional, so their wait counts are exact.
       const char *after = nxt ? nxt : apk;
@@ -771,12 +787,12 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       // loads issued in one burst per chunk stall the waves behind the vector-memory
       // queue, spread out they overlap the MFMAs. The loads are unconditional (also
       // in the skipped padding steps), so every fragment of the next chunk arrives.
-      auto run_chunk_il = [&](int c, const AFrag<KIND>(&a)[PF][RP], const char *nbase,
+      auto run_chunk_il = [&](int c, const AFrag<KIND>(&a)[PF][RP], const char *nbase, unsigned nlane,
                               AFrag<KIND>(&na)[PF][RP]) {
         const int g0 = chunk_g0(c), gn = chunk_g0(c + 1 < nchunks ? c + 1 : 0);
 #pragma unroll
         for (int s = 0; s < PF; ++s) {
-          load_step(nbase, s, na[s]);
+          load_step(nbase, nlane, s, na[s]);
           __builtin_amdgcn_sched_barrier(0);
           if (c * PF + s < ksteps) {
             if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
@@ -791,13 +807,14 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       int c = 0;
       for (; c + 1 < nchunks; c += 2) {
         if constexpr (KIND == K_I8 || KIND == K_BF16) {  // measured faster interleaved
-          run_chunk_il(c, fa[0], apk + (c + 1) * CHUNK, fa[1]);
-          run_chunk_il(c + 1, fa[1], c + 2 < nchunks ? apk + (c + 2) * CHUNK : after, fa[0]);
+          run_chunk_il(c, fa[0], apk + (c + 1) * CHUNK, alane, fa[1]);
+          run_chunk_il(c + 1, fa[1], c + 2 < nchunks ? apk + (c + 2) * CHUNK : after,
+                       c + 2 < nchunks ? alane : after_lane, fa[0]);
         } else {  // FP32 (4 MFMAs per fragment): one burst per chunk measured no slower
-          load_chunk(apk + (c + 1) * CHUNK, fa[1]);
+          load_chunk(apk + (c + 1) * CHUNK, alane, fa[1]);
           __builtin_amdgcn_sched_barrier(0);
           run_chunk(c, fa[0]);
-          load_chunk(c + 2 < nchunks ? apk + (c + 2) * CHUNK : after, fa[0]);
+          load_chunk(c + 2 < nchunks ? apk + (c + 2) * CHUNK : after, c + 2 < nchunks ? alane : after_lane, fa[0]);
           __builtin_amdgcn_sched_barrier(0);
           run_chunk(c + 1, fa[1]);
         }
@@ -805,9 +822,10 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       if (c < nchunks) {  // odd chunk count: the next loads wait for this chunk
         run_chunk(c, fa[0]);
         __builtin_amdgcn_sched_barrier(0);
-        load_chunk(after, fa[0]);
+        load_chunk(after, after_lane, fa[0]);
       }
       apk = after;
+      alane = after_lane;
       if (KIND == K_F32) {
 #pragma unroll
         for (int i = 0; i < RP; ++i)
@@ -821,7 +839,8 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     }
     if ((long)wave * RP * 32 >= Mp && next_gemm) {  // no rows here: fetch the next GEMM's first chunk
       apk = next_gemm;
-      load_chunk(apk, fa[0]);
+      alane = next_lane;
+      load_chunk(apk, alane, fa[0]);
     }
     if (last) break;  // C is written; nothing in LDS is read again
     if (!P.hold) {
@@ -847,10 +866,10 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <int KIND, int TNB, bool IMG, bool EPX, int G>
+template <int KIND, int TNB, bool IMG, bool EPX, int G, bool RAWA = false>
 __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   extern __shared__ uint4 arena[];
-  chain_tile<KIND, TNB, IMG, EPX, G>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
+  chain_tile<KIND, TNB, IMG, EPX, G, RAWA>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
 }
 
 constexpr int RP_of(int kind) {
@@ -908,6 +927,11 @@ int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *x
   }
   *hold = tile_n <= 0 || arena_bytes(kind, (int)wide, tile_n) > budget;
   if (!*hold) S = wide;
+  // A pass (PASS rows) can be shorter than a chunk (waves split over column groups),
+  // so the held bound Kc_i + (passes_i - 1) * PASS need not be a whole number of
+  // chunks; S must be, or a region starting on a chunk boundary could run past S
+  // inside a chunk (its reads are one base plus immediate offsets, never wrapped).
+  S = rc(S);
   long x = 0;
   for (int i = 0; i < n; ++i) {
     xbase[i] = (int)x;
@@ -929,6 +953,14 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   static_assert(wave_groups(KIND, TNB * 32) == G, "tile width and column groups disagree");
   auto kfn = c.img ? (c.epi ? chain_kernel<KIND, TNB, true, true, G> : chain_kernel<KIND, TNB, true, false, G>)
                    : (c.epi ? chain_kernel<KIND, TNB, false, true, G> : chain_kernel<KIND, TNB, false, false, G>);
+  if (c.raw_a) {  // row-major FP32 weights (chain_raw_weights_ok), matrix B, plain epilogues
+    if constexpr (KIND == K_F32) {
+      if (c.img || c.epi) return hipErrorInvalidValue;
+      kfn = chain_kernel<K_F32, TNB, false, false, G, true>;
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid((unsigned)((c.N + TN - 1) / TN));
@@ -996,6 +1028,13 @@ hipError_t launch_pack_t(PackJob J, hipStream_t s) {
 }
 
 }  // namespace
+
+bool chain_raw_weights_ok(const ChainArgs &c) {
+  if (c.kind != K_F32 || c.img || c.epi) return false;
+  for (int i = 0; i < c.num_gemm; ++i)
+    if (c.M[i] % 32 != 0 || c.K[i] % CS_of(K_F32) != 0 || ((uintptr_t)c.A[i] & 15) != 0) return false;
+  return true;
+}
 
 size_t chain_epi_lds_bytes(const long *M, int num_gemm) {
   long rows = 0;

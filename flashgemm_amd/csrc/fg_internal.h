@@ -232,7 +232,12 @@ struct ChainArgs {
   int tile_n;               // columns per workgroup
   const ImgGeom *img;       // nullptr: B is a matrix; else B is gathered from the image
   const EpiParams *epi;     // nullptr: the reference's epilogues; else per-GEMM extensions
+  int raw_a;                // A[i] are the row-major FP32 weights, read unpacked (chain_raw_weights_ok)
 };
+// FP32 chains whose weights need no padding (every M_i % 32 == 0, every K_i a whole
+// number of 64-k chunks, 16-byte aligned) on a matrix B with the plain epilogues: the
+// fused kernel reads A_i row-major in fragment order and the per-call pack is skipped.
+bool chain_raw_weights_ok(const ChainArgs &c);
 
 // Device-side description of one chain for the fused kernels (filled on the host
 // by make_chain_params; copied by value or read from a table in device memory).

@@ -321,6 +321,59 @@ def test_prepacked_weights_same_bits(gpu, dtype, f32_mode):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize("tile_n", [32, 64, 96, 128, 256])
+@pytest.mark.parametrize("K1,N,Ms", [(64, 1024, [128, 64, 96]), (256, 768, [256, 256]), (128, 512, [64]),
+                                     (64, 2048, [64, 64, 32])])
+def test_f32_row_major_weights_same_bits(gpu, K1, N, Ms, tile_n):
+    """FP32 chains whose weights need no padding (every M_i % 32 == 0, every K_i % 64 ==
+    0) skip the per-call pack: the fused kernel reads the row-major weights in fragment
+    order, across row passes and GEMM boundaries (K differs from GEMM to GEMM here).
+    Same bits as the same chain on weights packed beforehand, at every tile width the
+    shape fits (the last shape fits all of them); where it does not, both calls refuse."""
+    import torch
+    print(f"f32 row-major weights K1={K1} N={N} Ms={Ms} tile_n={tile_n}", flush=True)
+    rng = np.random.default_rng(K1 + N + tile_n)
+    Ks = [K1] + Ms[:-1]
+    B, As = rand_chain(rng, fg.DTYPE_F32, K1, N, Ms)
+    packed = [empty((fg.packed_weight_bytes(fg.DTYPE_F32, m, k),), torch.uint8) for m, k in zip(Ms, Ks)]
+    for a, m, k, p in zip(As, Ms, Ks, packed):
+        fg.pack_weight(fg.DTYPE_F32, dev(a), m, k, p)
+    C = empty((Ms[-1], N), torch.float32)
+    try:
+        got = run_chain(fg.DTYPE_F32, B, As, fuse=fg.FUSE_ON, tile_n=tile_n)
+    except fg.FlashgemmError as e:
+        assert e.status == fg.ERR_UNSUPPORTED and Ms != [64, 64, 32]
+        with pytest.raises(fg.FlashgemmError):
+            fg.chain(fg.DTYPE_F32, C, dev(B), packed, N=N, Ms=Ms, Ks=Ks, o=fg.opts(tile_n=tile_n, a_packed=1))
+        return
+    fg.chain(fg.DTYPE_F32, C, dev(B), packed, N=N, Ms=Ms, Ks=Ks, o=fg.opts(tile_n=tile_n, a_packed=1))
+    torch.cuda.synchronize()
+    want = C.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"{(got != want).sum()} of {got.size} differ"
+    assert O.rel_fro(got, O.chain_f32(B, As)) <= TOL_F32
+
+
+@pytest.mark.parametrize("dtype,K1,Ms", [(fg.DTYPE_F32, 64, [128, 64, 96]), (fg.DTYPE_BF16, 128, [256, 64])])
+def test_wide_tile_arena_whole_chunks(gpu, dtype, K1, Ms):
+    """256-column tiles walk each GEMM in passes shorter than a chunk; when their final
+    passes are held in registers the arena bound Kc + (passes - 1) * pass is not a whole
+    number of chunks, and an arena of that size let a chunk of X run past its end. The
+    arena is a whole number of chunks: these chains then need more than the 160 KB of
+    LDS at 256 columns, so the forced width is refused, and the planner's own choice
+    matches the oracle."""
+    print(f"wide-tile arena dtype={dtype} K1={K1} Ms={Ms}", flush=True)
+    rng = np.random.default_rng(K1)
+    B, As = rand_chain(rng, dtype, K1, 1024, Ms)
+    with pytest.raises(fg.FlashgemmError) as e:
+        run_chain(dtype, B, As, fuse=fg.FUSE_ON, tile_n=256)
+    assert e.value.status == fg.ERR_UNSUPPORTED
+    got = run_chain(dtype, B, As)
+    if dtype == fg.DTYPE_F32:
+        assert O.rel_fro(got, O.chain_f32(B, As)) <= TOL_F32
+    else:
+        assert O.rel_fro(got, O.chain_bf16(B, As, O.LAYOUT_MATH)) <= TOL_BF16
+
+
 @pytest.mark.parametrize("threads,N,Ms", [(1, 96, [24, 36, 12]), (4, 160, [12, 24, 36]), (2, 64, [36, 48, 24, 12]),
                                           (3, 96, [24, 12])])
 def test_int8_reference_layout_device_api(gpu, threads, N, Ms):

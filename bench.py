@@ -36,10 +36,10 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "int8": 5033.2, "fp64": 78.6}  # dense TFLOP/s (TOPS)
 HBM_GBS = 8000.0
-# HBM bytes per launch of the main chain kernel: rocprofv3 FETCH_SIZE 6217.5 KiB x 1.9997
+# HBM bytes per launch of the main chain kernel: rocprofv3 FETCH_SIZE 6214.0 KiB x 1.9997
 # + WRITE_SIZE 8192.0 KiB x 1.0000, the scales being what the same counters read on a
 # plain 256 MiB device copy (bench.py --copy-calibration). Algorithmic bytes: 17.30 MB.
-TRAFFIC_MAIN = 21120008
+TRAFFIC_MAIN = 21112871
 
 
 def log(msg: str) -> None:

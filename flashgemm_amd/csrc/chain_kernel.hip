@@ -649,7 +649,11 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   constexpr size_t STEP = RAW ? (size_t)T::KC * 4 : (size_t)RP * FRAG;  // bytes per k-step
   AFrag<KIND> fa[2][PF][RP];
   // the RP fragments of k-step s of the chunk at base
+  // (packed: the carried lane offset is not read, so it costs no register; a carried
+  // copy of lane16 the compiler cannot prove constant made the INT8 64-column kernel
+  // spill)
   auto load_step = [&](const char *base, unsigned lo, int s, AFrag<KIND>(&dst)[RP]) {
+    if (!RAW) lo = lane16;
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
       const char *p = base + (size_t)s * STEP + (size_t)i * FRAG;

@@ -8,6 +8,7 @@
 // a HIP stream. Host-pointer calls copy in and out around that.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -85,6 +86,17 @@ int scratch(hipStream_t s, const char *name, size_t bytes, void **out) {
   return FG_OK;
 }
 
+// Frees every scratch buffer of stream s (the caller has synchronized s; called
+// before a stream the library created is destroyed, so its buffers do not outlive it).
+void release_stream_scratch(hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_scratch.find(s);
+  if (it == g_scratch.end()) return;
+  for (auto &kv : it->second.bufs)
+    if (kv.second.first) (void)hipFree(kv.second.first);
+  g_scratch.erase(it);
+}
+
 size_t dtype_bytes(int dtype) { return dtype == FG_DTYPE_F32 ? 4 : dtype == FG_DTYPE_BF16 ? 2 : 1; }
 
 int kind_of(int dtype, int f32_mode) {
@@ -131,6 +143,28 @@ static double fused_cost(int kind, long N, int num_gemm, const long *M, const lo
   const long tiles = (N + tn - 1) / tn;
   const long rounds = (tiles + num_cus() - 1) / num_cus();
   return (double)rounds * t;
+}
+
+// Upper bound of the per-stream scratch one chain can take when it does not run in a
+// grouped launch: the two spill buffers of the unfused path, the largest split-K slab
+// set (one slab per split, skinny FP32 GEMMs included; none when single_split), the
+// INT8 column sums, and the packed weights of a fused run.
+size_t chain_scratch_bound(int dtype, int f32_mode, long N, int n, const long *M, const long *K, bool single_split) {
+  const int kind = kind_of(dtype, f32_mode);
+  long maxm = 0;
+  for (int i = 0; i + 1 < n; ++i) maxm = M[i] > maxm ? M[i] : maxm;
+  size_t bytes = (size_t)2 * maxm * N * dtype_bytes(dtype), part = 0, packed = 0;
+  for (int i = 0; i < n; ++i) {
+    long sk = 1;
+    if (!single_split) {
+      long kper;
+      sk = (kind == K_F32 && skinny_f32_shape(M[i], N)) ? skinny_f32_splits(M[i], N, K[i], &kper)
+                                                         : plan_splits(M[i], N, K[i], kind);
+    }
+    if (sk > 1) part = std::max(part, (size_t)sk * M[i] * N * 4);
+    packed += (packed_a_bytes(kind, M[i], K[i]) + 255) & ~(size_t)255;
+  }
+  return bytes + part + packed + (kind == K_I8 ? (size_t)N * 4 : 0);
 }
 
 // Fused tile width: the cheapest tile (fused_cost) that fits LDS; within 3% the one
@@ -270,8 +304,10 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   *splits = o.splits;
   if (!ref && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
     int f, t, sp;
+    // an entry that cannot run this exact shape (a tile width the kind lacks, or an
+    // arena beyond LDS) is ignored: the heuristics below decide instead
     if (autotune_lookup(kind, N, num_gemm, M, K, &f, &t, &sp) &&
-        (!f || (chain_fused_shape_ok(kind, N, num_gemm, K) &&
+        (!f || (chain_fused_tile_ok(kind, t) && chain_fused_shape_ok(kind, N, num_gemm, K) &&
                 chain_fused_lds_bytes(kind, M, K, num_gemm, t, epi) <= LDS_BYTES_MAX))) {
       *fused = f;
       *tile_n = f ? t : 0;

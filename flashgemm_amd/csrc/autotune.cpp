@@ -196,7 +196,8 @@ int fg_autotune_load(const char *path) {
     for (int i = 0; i < k.num_gemm && ok; ++i) ok = fscanf(f, "%ld", &k.K[i]) == 1;
     Choice c;
     ok = ok && fscanf(f, "%d %d %d %lf", &c.fused, &c.tile_n, &c.splits, &c.ms) == 4;
-    if (!ok || (c.fused && c.tile_n != 32 && c.tile_n != 64 && c.tile_n != 96 && c.tile_n != 128)) {
+    // a fused entry names a tile width the fused kernels have for this kind
+    if (!ok || (c.fused && !chain_fused_tile_ok(k.kind, c.tile_n)) || (!c.fused && c.splits < 0)) {
       rc = set_error(FG_ERR_INVALID, "%s: bad entry", path);
       break;
     }

@@ -12,6 +12,11 @@ int ensure_init();
 int num_cus();
 // Named per-stream scratch buffer of at least `bytes` (grow-only).
 int scratch(hipStream_t s, const char *name, size_t bytes, void **out);
+// Frees the scratch buffers and the grouped-table ring of stream s (s synchronized).
+void release_stream_scratch(hipStream_t s);
+void release_stream_ring(hipStream_t s);
+// Upper bound of the scratch a chain takes outside a grouped launch (api.cpp).
+size_t chain_scratch_bound(int dtype, int f32_mode, long N, int n, const long *M, const long *K, bool single_split);
 int kind_of(int dtype, int f32_mode);
 int plan_splits(long M, long N, long K, int kind);
 int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bool epi = false);

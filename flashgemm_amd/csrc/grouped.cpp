@@ -213,6 +213,29 @@ int grouped_grid(int items) { return std::max(1, std::min(items, num_cus())); }
 
 }  // namespace
 
+namespace fg {
+void release_stream_ring(hipStream_t s) {
+  StreamRing *R = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_ring_mu);
+    for (size_t i = 0; i < g_rings.size(); ++i)
+      if (g_rings[i].first == s) {
+        R = g_rings[i].second;
+        g_rings.erase(g_rings.begin() + i);
+        break;
+      }
+  }
+  if (!R) return;
+  for (Slot &sl : R->slot) {
+    if (sl.done) (void)hipEventSynchronize(sl.done);
+    if (sl.host) (void)hipHostFree(sl.host);
+    if (sl.dev) (void)hipFree(sl.dev);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
+  delete R;
+}
+}  // namespace fg
+
 struct fg_grouped_graph {
   struct G {
     Slot slot;

@@ -21,6 +21,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <sys/prctl.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -69,7 +70,8 @@ struct fg_server {
   size_t weight_bytes = 0;
 
   std::mutex mu;
-  std::condition_variable cv;
+  std::condition_variable cv;       // drain(): a batch completed
+  std::condition_variable work_cv;  // scheduler: a request was submitted, or stop
   std::deque<long> pending;          // submitted, not admitted
   std::vector<long> admitted;        // admitted, not launched
   std::vector<Request> reqs;         // by id
@@ -82,6 +84,7 @@ struct fg_server {
 
   std::vector<hipStream_t> streams;
   std::vector<fg_grouped_graph *> graphs;
+  long graph_tiles = 0;  // tile capacity of each captured graph's table
   std::vector<hipEvent_t> free_events;
   int next_stream = 0;
   std::atomic<bool> stop{false};
@@ -188,9 +191,16 @@ bool step(fg_server *S) {
     }
   } else {
     std::vector<fg_chain_req> fit, rest;
-    for (const fg_chain_req &x : q) (x.a_packed ? fit : rest).push_back(x);
+    long tiles = 0;
+    for (const fg_chain_req &x : q) {
+      (x.a_packed ? fit : rest).push_back(x);
+      if (x.a_packed) tiles += (x.N + grouped_tile_n(kind_of(x.dtype, x.f32_mode)) - 1) /
+                               grouped_tile_n(kind_of(x.dtype, x.f32_mode));
+    }
     if (!fit.empty()) {
-      if (S->cfg.use_graph)
+      // a batch of very wide requests can hold more tiles than the captured graph's
+      // table: it is launched directly instead
+      if (S->cfg.use_graph && tiles <= S->graph_tiles)
         rc = fg_grouped_graph_launch(S->graphs[sidx], (int)fit.size(), fit.data(), st);
       else
         rc = fg_grouped_chain((int)fit.size(), fit.data(), st);
@@ -210,10 +220,23 @@ bool step(fg_server *S) {
   return true;
 }
 
+// The scheduler sleeps instead of spinning: with nothing submitted and nothing on the
+// device it waits for a submit; with batches in flight it wakes every POLL to look
+// for completed ones (timer slack 1 us, so the short waits are not rounded up to the
+// default 50 us) or at once when a request arrives.
+constexpr auto POLL = std::chrono::microseconds(10);
+
 void worker_main(fg_server *S) {
   (void)hipSetDevice(0);
+  (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
   while (!S->stop.load()) {
-    if (!step(S)) std::this_thread::yield();
+    if (step(S)) continue;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (S->stop.load()) break;
+    if (S->inflight.empty() && S->admitted.empty() && S->pending.empty())
+      S->work_cv.wait(lk, [&] { return S->stop.load() || !S->pending.empty(); });
+    else
+      S->work_cv.wait_for(lk, POLL);
   }
 }
 
@@ -246,7 +269,8 @@ int fg_server_create(const fg_server_config *cfg, fg_server **out) {
     if (!rc) S->streams.push_back(s);
     if (!rc && S->cfg.use_graph && !S->cfg.per_gemm_baseline) {
       fg_grouped_graph *g = nullptr;
-      rc = fg_grouped_graph_create(S->cfg.max_batch, S->cfg.max_batch * 160, &g);
+      S->graph_tiles = (long)S->cfg.max_batch * 160;
+      rc = fg_grouped_graph_create(S->cfg.max_batch, (int)S->graph_tiles, &g);
       S->graphs.push_back(g);
     }
   }
@@ -320,12 +344,32 @@ int fg_server_submit(fg_server *S, int set_id, long N, const void *B, void *C, l
   r.B = B;
   r.C = C;
   r.bytes = (size_t)w.K[0] * N * e + (size_t)w.M[w.num_gemm - 1] * N * 4;
+  // A request that cannot ride a grouped launch runs alone and takes library scratch
+  // (spill buffers, split-K slabs, packed weights): that is charged too.
+  fg_chain_req x;
+  memset(&x, 0, sizeof x);
+  x.dtype = w.dtype;
+  x.f32_mode = w.f32_mode;
+  x.C = C;
+  x.B = B;
+  x.N = N;
+  x.num_gemm = w.num_gemm;
+  for (int g = 0; g < w.num_gemm; ++g) {
+    x.A[g] = w.A[g];
+    x.M[g] = w.M[g];
+    x.K[g] = w.K[g];
+  }
+  const int fits = S->cfg.per_gemm_baseline ? 0 : fg_grouped_fits(&x);
+  if (fits < 0) return fits;
+  if (!fits)
+    r.bytes += chain_scratch_bound(w.dtype, w.f32_mode, N, w.num_gemm, w.M, w.K, S->cfg.per_gemm_baseline != 0);
   if (r.bytes + S->weight_bytes > S->cfg.hbm_budget_bytes)
     return set_error(FG_ERR_BUDGET, "request of %zu bytes can never fit the budget", r.bytes);
   r.arrival = Clock::now();
   r.status = FG_OK;
   S->reqs.push_back(r);
   S->pending.push_back(r.id);
+  S->work_cv.notify_one();
   if (req_id) *req_id = r.id;
   return FG_OK;
 }
@@ -376,13 +420,22 @@ long fg_server_num_requests(fg_server *S) {
 
 void fg_server_destroy(fg_server *S) {
   if (!S) return;
-  S->stop.store(true);
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->stop.store(true);
+    S->work_cv.notify_all();
+  }
   if (S->worker.joinable()) S->worker.join();
   for (auto s : S->streams) (void)hipStreamSynchronize(s);
   for (auto &b : S->inflight) (void)hipEventDestroy(b.ev);
   for (auto e : S->free_events) (void)hipEventDestroy(e);
   for (auto g : S->graphs) fg_grouped_graph_destroy(g);
-  for (auto s : S->streams) (void)hipStreamDestroy(s);
+  for (auto s : S->streams) {
+    // the library's per-stream scratch and table ring die with the stream
+    release_stream_scratch(s);
+    release_stream_ring(s);
+    (void)hipStreamDestroy(s);
+  }
   for (auto &w : S->sets)
     for (int i = 0; i < w.num_gemm; ++i)
       if (w.packed[i]) (void)hipFree(w.packed[i]);

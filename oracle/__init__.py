@@ -50,6 +50,8 @@ def lib():
         L.oracle_sgemm.restype = i
         L.oracle_bf16_rne.argtypes = [ctypes.c_float]
         L.oracle_bf16_rne.restype = ctypes.c_uint16
+        L.oracle_bf16_trunc.argtypes = [ctypes.c_float]
+        L.oracle_bf16_trunc.restype = ctypes.c_uint16
         L.oracle_partition.argtypes = [l, i, lp, lp, lp]
         _lib = L
     return _lib
@@ -139,6 +141,16 @@ def bf16_rne(x: np.ndarray) -> np.ndarray:
     out = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
     out[nan] = ((u[nan] >> 16) | 0x40).astype(np.uint16)
     return out
+
+
+def bf16_trunc_scalar(x: float) -> int:
+    """oracle_bf16_trunc: the reference generator's float_to_bf16 (multigemm/src/utils.h:20-33),
+    which keeps sign, exponent and the top 7 mantissa bits (truncation)."""
+    return int(lib().oracle_bf16_trunc(float(x)))
+
+
+def bf16_rne_scalar(x: float) -> int:
+    return int(lib().oracle_bf16_rne(float(x)))
 
 
 def bf16_to_f32(b: np.ndarray) -> np.ndarray:

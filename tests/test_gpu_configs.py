@@ -1,0 +1,338 @@
+"""GPU parity at the benchmark workloads themselves (full sizes, the benchmark's own
+seeded data), and the GPU FP64 oracle that the precision figures rest on.
+
+Full-size outputs are checked where the oracle finishes in seconds: on sampled rows
+(a GEMM's rows are independent) or sampled columns (in the mathematical layout every
+column of a chain's C depends only on its column of B), plus size-independent
+properties (fused = unfused bit for bit for INT8, repeated calls bit-identical,
+single-request results identical inside a serving mix). Bars, written per test:
+INT8 bit-exact; FP32 relative Frobenius <= 1e-5 against the oracle and FP64, and the
+reference's own |C - C_ref| <= 1e-3 for single GEMMs (singlegemm/FP32_ID23.c:125-128);
+split-bf16 x3 <= 1e-4 against FP64; BF16 chains <= TOL_BF16 against the oracle and
+<= TOL_BF16_F64 against the FP64 chain of the same bf16 inputs (the intermediates'
+bf16 rounding is part of the algorithm, so it is inside that error). Every measured
+error is printed.
+"""
+import numpy as np
+import pytest
+
+import flashgemm_amd as fg
+from flashgemm_amd import workloads as W
+import oracle as O
+from test_gpu_parity import TOL_BF16, TOL_F32, TOL_X3, empty
+
+pytestmark = pytest.mark.gpu
+
+TOL_BF16_F64 = 2e-2
+TOL_F64_ORACLE = 1e-12
+
+
+def report(what, err, tol):
+    print(f"  {what}: {err:.3e} (bar {tol:.0e})", flush=True)
+    assert err <= tol, f"{what}: {err:.3e} > {tol:.0e}"
+
+
+def host(t):
+    import torch
+    t = t.contiguous()
+    if t.dtype == torch.bfloat16:
+        return t.view(torch.int16).cpu().numpy().view(np.uint16)
+    return t.cpu().numpy()
+
+
+def f64_of_bf16(bits):
+    return O.bf16_to_f32(bits).astype(np.float64)
+
+
+# ------------------------------------------- FP32 GEMM 2048^3 (singlegemm) --
+
+@pytest.mark.parametrize("mode", [fg.F32_MFMA, fg.F32_BF16X3])
+def test_sgemm_2048_sampled_rows(gpu, mode):
+    import torch
+    M = N = K = 2048
+    print(f"sgemm 2048^3 mode={mode}", flush=True)
+    A, B = W.sgemm_operands(torch, M, N, K)
+    C = empty((M, N), torch.float32)
+    fg.sgemm(C, A, B, M, N, K, f32_mode=mode)
+    torch.cuda.synchronize()
+    rows = np.sort(np.random.default_rng(1).choice(M, 64, replace=False))
+    Ah, Bh = host(A)[rows], host(B)
+    got = host(C)[rows]
+    f64 = Ah.astype(np.float64) @ Bh.astype(np.float64)
+    if mode == fg.F32_MFMA:
+        want = O.sgemm(Ah, Bh)
+        report("max |C - oracle|", float(np.abs(got - want).max()), 1e-3)
+        report("rel vs oracle", O.rel_fro(got, want), TOL_F32)
+        report("rel vs fp64", O.rel_fro(got, f64), TOL_F32)
+    else:
+        report("bf16x3 rel vs fp64", O.rel_fro(got, f64), TOL_X3)
+
+
+def test_sgemm_2048_reference_entry_point(gpu):
+    """LibShalom_sgemm_mp on host arrays (the reference's dispatcher signature,
+    singlegemm/FP32_ID23.c:35-62) at its benchmark size, with its own check."""
+    M = N = K = 2048
+    rng = np.random.default_rng(2048)
+    A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
+    B = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    C = np.zeros((M, N), np.float32)
+    fg.LibShalom_sgemm_mp(0, 0, C, A, B, M, N, K)
+    rows = np.sort(rng.choice(M, 64, replace=False))
+    want = O.sgemm(A[rows], B)
+    report("max |C - oracle|", float(np.abs(C[rows] - want).max()), 1e-3)
+    report("rel vs oracle", O.rel_fro(C[rows], want), TOL_F32)
+
+
+# ----------------------------------------- FP32 skinny GEMMs (singlegemm) --
+
+@pytest.mark.parametrize("M,N,K", [(8192, 64, 8192), (64, 64, 65536)])
+def test_sgemm_skinny_full_size(gpu, M, N, K):
+    import torch
+    print(f"skinny sgemm {M}x{N}x{K}", flush=True)
+    A, B = W.sgemm_operands(torch, M, N, K)
+    outs = []
+    for _ in range(2):
+        C = empty((M, N), torch.float32)
+        fg.sgemm(C, A, B, M, N, K)
+        torch.cuda.synchronize()
+        outs.append(host(C))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32)), "not deterministic"
+    rows = np.arange(M) if M <= 64 else np.sort(np.random.default_rng(2).choice(M, 64, replace=False))
+    Ah, Bh = host(A)[rows], host(B)
+    got = outs[0][rows]
+    want = O.sgemm(Ah, Bh)
+    report("max |C - oracle|", float(np.abs(got - want).max()), 1e-3)
+    report("rel vs oracle", O.rel_fro(got, want), TOL_F32)
+    report("rel vs fp64", O.rel_fro(got, Ah.astype(np.float64) @ Bh.astype(np.float64)), TOL_F32)
+
+
+# ------------------------------------------------ FP32 2-GEMM chain, N = 8192 --
+
+def test_f32_chain2_n8192(gpu):
+    import torch
+    print("fp32 chain 256->256->256, N=8192", flush=True)
+    B, As = W.chain2_f32(torch)
+    N = B.shape[1]
+    C = empty((As[-1].shape[0], N), torch.float32)
+    fg.chain(fg.DTYPE_F32, C, B, As)
+    torch.cuda.synchronize()
+    cols = np.sort(np.random.default_rng(3).choice(N, 64, replace=False))
+    Bh, Ah = host(B)[:, cols], [host(a) for a in As]
+    got = host(C)[:, cols]
+    report("rel vs oracle", O.rel_fro(got, O.chain_f32(np.ascontiguousarray(Bh), Ah)), TOL_F32)
+    report("rel vs fp64", O.rel_fro(got, O.chain_f64(Bh.astype(np.float64), [a.astype(np.float64) for a in Ah])),
+           TOL_F32)
+    # the reference's entry point on host copies: the same plan, the same bits
+    Cl = np.zeros((As[-1].shape[0], N), np.float32)
+    Bfull = host(B)
+    fg.flashgemm_multi_f32f32f32(Cl, Bfull, N, 2, Ah[0], 256, 256, Ah[1], 256, 256)
+    assert np.array_equal(Cl.view(np.uint32), host(C).view(np.uint32))
+
+
+# ------------------------------------------------- INT8 / BF16 3-GEMM chains --
+
+def test_int8_chain3_full_range_full_size(gpu):
+    """The benchmark's INT8 3-chain with its [-127, 127] weights (most intermediates
+    saturate): fused = unfused over the whole output, sampled columns = oracle."""
+    import torch
+    print("int8 3-chain N=16384, full-range weights", flush=True)
+    B, As = W.chain3(torch, fg.DTYPE_INT8)
+    N = B.shape[1]
+    C = empty((512, N), torch.int32)
+    Cu = empty((512, N), torch.int32)
+    fg.chain(fg.DTYPE_INT8, C, B, As)
+    fg.chain(fg.DTYPE_INT8, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF))
+    torch.cuda.synchronize()
+    assert torch.equal(C, Cu), "fused and unfused differ"
+    cols = np.sort(np.random.default_rng(4).choice(N, 64, replace=False))
+    want = O.chain_int8(np.ascontiguousarray(host(B)[:, cols]), [host(a) for a in As], O.LAYOUT_MATH)
+    got = host(C)[:, cols]
+    assert np.array_equal(got, want), f"{(got != want).sum()} of {got.size} differ"
+    print(f"  saturated share of the first intermediate's sampled columns: "
+          f"{np.mean(np.abs(host(As[0]).view(np.uint8).astype(np.int64) @ host(B)[:, cols].astype(np.int64)) > 127):.3f}")
+
+
+@pytest.mark.parametrize("threads", [1, 4, 16])
+def test_int8_chain3_drop_in_reference_layout(gpu, threads):
+    """flashgemm_multi_int8uint8int32 at N = 16384 on host arrays, in the reference's
+    intermediate layout for `threads` threads (every panel 32 wide and aligned);
+    sampled whole panels bit for bit against the oracle's layout."""
+    import torch
+    print(f"int8 3-chain drop-in, reference layout, {threads} threads", flush=True)
+    B, As = W.chain3(torch, fg.DTYPE_INT8, seed=7)
+    Bh, Ah = host(B), [host(a) for a in As]
+    N = Bh.shape[1]
+    C = np.zeros((512, N), np.int32)
+    fg.flashgemm_set_thread_num(threads)
+    try:
+        fg.flashgemm_multi_int8uint8int32(C, Bh, N, 3, Ah[0], 512, 512, Ah[1], 1024, 512, Ah[2], 512, 1024)
+    finally:
+        fg.flashgemm_set_thread_num(1)
+    panels = np.sort(np.random.default_rng(threads).choice(N // 32, 4, replace=False))
+    cols = (panels[:, None] * 32 + np.arange(32)[None, :]).ravel()
+    want = O.chain_int8(np.ascontiguousarray(Bh[:, cols]), Ah, O.LAYOUT_REFERENCE, 1)
+    assert np.array_equal(C[:, cols], want), f"{(C[:, cols] != want).sum()} differ"
+
+
+def test_bf16_chain3_full_size(gpu):
+    import torch
+    print("bf16 3-chain N=16384", flush=True)
+    B, As = W.chain3(torch, fg.DTYPE_BF16)
+    N = B.shape[1]
+    C = empty((512, N), torch.float32)
+    Cu = empty((512, N), torch.float32)
+    fg.chain(fg.DTYPE_BF16, C, B, As)
+    fg.chain(fg.DTYPE_BF16, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF))
+    torch.cuda.synchronize()
+    cols = np.sort(np.random.default_rng(5).choice(N, 64, replace=False))
+    Bh, Ah = np.ascontiguousarray(host(B)[:, cols]), [host(a) for a in As]
+    got = host(C)[:, cols]
+    report("fused rel vs oracle", O.rel_fro(got, O.chain_bf16(Bh, Ah, O.LAYOUT_MATH)), TOL_BF16)
+    report("unfused rel vs oracle", O.rel_fro(host(Cu)[:, cols], O.chain_bf16(Bh, Ah, O.LAYOUT_MATH)), TOL_BF16)
+    f64 = O.chain_f64(f64_of_bf16(Bh), [f64_of_bf16(a) for a in Ah])
+    report("fused rel vs fp64", O.rel_fro(got, f64), TOL_BF16_F64)
+    report("fused vs unfused", O.rel_fro(host(C), host(Cu)), TOL_BF16)
+
+
+# ------------------------------------------------- FP32 precision 4-chain --
+
+def test_precision_chain_full_size_and_f64_oracle(gpu):
+    """The precision study at its shapes and scaled data: plain FP32 and split-bf16 x3
+    against the CPU FP64 chain on sampled columns; the GPU FP64 chain (the reference
+    every precision figure is measured against) against the CPU FP64 chain; and
+    fg_error_norms against numpy over the whole output."""
+    import torch
+    print("precision 4-chain N=16384, 1024->4096->1024->4096->1024", flush=True)
+    B, As = W.precision_chain(torch)
+    N = B.shape[1]
+    Ms = [a.shape[0] for a in As]
+    ref = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float64)
+    Bd, Ad = B.double(), [a.double() for a in As]  # alive until the queued chain has run
+    fg.f64_chain(ref, Bd, Ad)
+    outs = {}
+    for name, mode in (("fp32", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3)):
+        C = empty((Ms[-1], N), torch.float32)
+        fg.chain(fg.DTYPE_F32, C, B, As, o=fg.opts(f32_mode=mode))
+        outs[name] = C
+    torch.cuda.synchronize()
+    cols = np.sort(np.random.default_rng(6).choice(N, 64, replace=False))
+    Bh = np.ascontiguousarray(host(B)[:, cols]).astype(np.float64)
+    want = O.chain_f64(Bh, [host(a).astype(np.float64) for a in As])
+    report("GPU fp64 chain vs CPU fp64 chain", O.rel_fro(host(ref)[:, cols], want), TOL_F64_ORACLE)
+    report("fp32 rel vs fp64", O.rel_fro(host(outs["fp32"])[:, cols], want), TOL_F32)
+    report("bf16x3 rel vs fp64", O.rel_fro(host(outs["bf16x3"])[:, cols], want), TOL_X3)
+    refh = host(ref)
+    for name, C in outs.items():
+        d2, r2 = fg.error_norms(0, C, ref, C.numel())
+        x = host(C).astype(np.float64)
+        nd2, nr2 = float(((x - refh) ** 2).sum()), float((refh ** 2).sum())
+        report(f"error_norms {name}: d2", abs(d2 - nd2) / nd2, 1e-9)
+        report(f"error_norms {name}: r2", abs(r2 - nr2) / nr2, 1e-12)
+
+
+# ------------------------------------------------------- GPU FP64 oracle --
+
+def test_f64_chain_ragged_vs_cpu(gpu):
+    """fg_f64_chain on a ragged 3-chain (tails in M, N and K against its 64 x 64 tiles
+    and 16-deep k stages) against the CPU FP64 chain and numpy."""
+    import torch
+    rng = np.random.default_rng(8)
+    K1, N, Ms = 50, 130, [70, 33, 97]
+    Ks = [K1] + Ms[:-1]
+    B = rng.standard_normal((K1, N))
+    As = [rng.standard_normal((m, k)) for m, k in zip(Ms, Ks)]
+    C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float64)
+    fg.f64_chain(C, torch.from_numpy(B).cuda(), [torch.from_numpy(a).cuda() for a in As])
+    torch.cuda.synchronize()
+    got = C.cpu().numpy()
+    report("vs oracle_chain_f64", O.rel_fro(got, O.chain_f64(B, As)), 1e-13)
+    report("vs numpy", O.rel_fro(got, As[2] @ (As[1] @ (As[0] @ B))), 1e-13)
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_error_norms_and_to_f64_vs_numpy(gpu, kind):
+    """fg_error_norms (sum (x - ref)^2, sum ref^2) and fg_to_f64 for fp32, bf16, int32
+    and fp64 inputs of a ragged length, against numpy in float64."""
+    import torch
+    rng = np.random.default_rng(9 + kind)
+    n = 300001
+    ref = rng.standard_normal(n) * 100
+    if kind == 0:
+        x = (ref + rng.standard_normal(n)).astype(np.float32)
+        xd, xf = torch.from_numpy(x).cuda(), x.astype(np.float64)
+    elif kind == 1:
+        bits = O.bf16_rne((ref + rng.standard_normal(n)).astype(np.float32))
+        xd, xf = torch.from_numpy(bits.view(np.int16)).cuda(), O.bf16_to_f32(bits).astype(np.float64)
+    elif kind == 2:
+        x = np.rint(ref + rng.standard_normal(n)).astype(np.int32)
+        xd, xf = torch.from_numpy(x).cuda(), x.astype(np.float64)
+    else:
+        x = ref + rng.standard_normal(n) * 1e-3
+        xd, xf = torch.from_numpy(x).cuda(), x
+    refd = torch.from_numpy(ref).cuda()
+    d2, r2 = fg.error_norms(kind, xd, refd, n)
+    nd2, nr2 = float(((xf - ref) ** 2).sum()), float((ref ** 2).sum())
+    report("d2", abs(d2 - nd2) / nd2, 1e-12)
+    report("r2", abs(r2 - nr2) / nr2, 1e-12)
+    out = torch.empty(n, device="cuda", dtype=torch.float64)
+    fg.to_f64(kind, xd, out, n)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), xf)
+
+
+# -------------------------------------------------------- serving trace --
+
+def test_serving_trace_1000_requests(gpu):
+    """1,000 requests of the serving workload (64 resident weight sets of 2-4 GEMMs,
+    widths to 2048, request widths log-uniform in 1..4096, 50/30/20 % BF16/INT8/FP32)
+    with Poisson arrivals at 20,000 requests/s through the engine: every result is bit
+    for bit the single-launch result of the same request (fg_chain alone), and 48
+    sampled requests match the oracle on sampled columns."""
+    import torch
+    sets = W.serving_sets(torch)
+    set_of, Ns, gaps = W.serving_trace(1000, len(sets))
+    arrivals = (np.cumsum(gaps) / 20000.0).tolist()
+    sizes = [sets[s]["w"][-1] * int(n) * 4 for s, n in zip(set_of, Ns)]
+    offs = np.concatenate([[0], np.cumsum([(b + 255) // 256 * 256 for b in sizes])])
+    arena = torch.zeros(int(offs[-1]), device="cuda", dtype=torch.uint8)
+    single = torch.zeros_like(arena)
+
+    def view(buf, i):
+        s = sets[set_of[i]]
+        dt = torch.int32 if s["dtype"] == fg.DTYPE_INT8 else torch.float32
+        return buf[int(offs[i]):int(offs[i]) + sizes[i]].view(dt).view(s["w"][-1], int(Ns[i]))
+
+    torch.cuda.synchronize()
+    srv = fg.Server()
+    try:
+        sids = [srv.add_weights(s["dtype"], s["As"]) for s in sets]
+        print("serving 1000 requests", flush=True)
+        srv.run_trace([sids[s] for s in set_of], [int(n) for n in Ns], [sets[s]["B"] for s in set_of],
+                      [arena.data_ptr() + int(o) for o in offs[:-1]], arrivals)
+        lat = np.array(srv.latencies())
+    finally:
+        srv.close()
+    assert lat.min() > 0
+    print(f"  p50 {np.percentile(lat, 50) * 1e3:.3f} ms, p99 {np.percentile(lat, 99) * 1e3:.3f} ms", flush=True)
+    for i, (s, n) in enumerate(zip(set_of, Ns)):
+        st = sets[s]
+        w = st["w"]
+        fg.chain(st["dtype"], view(single, i), st["B"], st["As"], N=int(n), Ms=w[1:], Ks=w[:-1])
+    torch.cuda.synchronize()
+    differ = [i for i in range(len(Ns)) if not torch.equal(view(arena, i), view(single, i))]
+    assert not differ, f"{len(differ)} requests differ from their single launch, first {differ[:5]}"
+    rng = np.random.default_rng(10)
+    for i in np.sort(rng.choice(len(Ns), 48, replace=False)):
+        st, n = sets[set_of[i]], int(Ns[i])
+        w = st["w"]
+        cols = np.sort(rng.choice(n, min(n, 8), replace=False))
+        Bfull = host(st["B"][:w[0] * n].view(w[0], n))
+        Bh = np.ascontiguousarray(Bfull[:, cols])
+        Ah = [host(a) for a in st["As"]]
+        got = host(view(arena, i))[:, cols]
+        if st["dtype"] == fg.DTYPE_INT8:
+            assert np.array_equal(got, O.chain_int8(Bh, Ah, O.LAYOUT_MATH)), f"request {i}"
+        elif st["dtype"] == fg.DTYPE_BF16:
+            assert O.rel_fro(got, O.chain_bf16(Bh, Ah, O.LAYOUT_MATH)) <= TOL_BF16, f"request {i}"
+        else:
+            assert O.rel_fro(got, O.chain_f32(Bh, Ah)) <= TOL_F32, f"request {i}"

@@ -21,7 +21,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <sys/prctl.h>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -220,23 +219,23 @@ bool step(fg_server *S) {
   return true;
 }
 
-// The scheduler sleeps instead of spinning: with nothing submitted and nothing on the
-// device it waits for a submit; with batches in flight it wakes every POLL to look
-// for completed ones (timer slack 1 us, so the short waits are not rounded up to the
-// default 50 us) or at once when a request arrives.
-constexpr auto POLL = std::chrono::microseconds(10);
-
+// The scheduler sleeps while the engine is idle (nothing submitted, nothing on the
+// device) until a submit or stop wakes it. While batches are in flight it keeps
+// polling their events, yielding between passes: a timed wait between polls (10 us,
+// timer slack 1 us) delayed each completion by its sleep and, at 20,000 requests/s,
+// raised the p99 latency from 3.5 to 14 ms.
 void worker_main(fg_server *S) {
   (void)hipSetDevice(0);
-  (void)prctl(PR_SET_TIMERSLACK, 1000UL, 0, 0, 0);
   while (!S->stop.load()) {
     if (step(S)) continue;
     std::unique_lock<std::mutex> lk(S->mu);
     if (S->stop.load()) break;
-    if (S->inflight.empty() && S->admitted.empty() && S->pending.empty())
+    if (S->inflight.empty() && S->admitted.empty() && S->pending.empty()) {
       S->work_cv.wait(lk, [&] { return S->stop.load() || !S->pending.empty(); });
-    else
-      S->work_cv.wait_for(lk, POLL);
+    } else {
+      lk.unlock();
+      std::this_thread::yield();
+    }
   }
 }
 

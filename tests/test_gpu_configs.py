@@ -8,7 +8,7 @@ properties (fused = unfused bit for bit for INT8, repeated calls bit-identical,
 single-request results identical inside a serving mix). Bars, written per test:
 INT8 bit-exact; FP32 relative Frobenius <= 1e-5 against the oracle and FP64, and the
 reference's own |C - C_ref| <= 1e-3 for single GEMMs (singlegemm/FP32_ID23.c:125-128);
-split-bf16 x3 <= 1e-4 against FP64; BF16 chains <= TOL_BF16 against the oracle and
+split-bf16 x3 <= 3e-5 against FP64; BF16 chains <= TOL_BF16 against the oracle and
 <= TOL_BF16_F64 against the FP64 chain of the same bf16 inputs (the intermediates'
 bf16 rounding is part of the algorithm, so it is inside that error). Every measured
 error is printed.
@@ -19,11 +19,11 @@ import pytest
 import flashgemm_amd as fg
 from flashgemm_amd import workloads as W
 import oracle as O
-from test_gpu_parity import TOL_BF16, TOL_F32, TOL_X3, empty
+from test_gpu_parity import assert_rel, TOL_BF16, TOL_F32, TOL_X3, empty
 
 pytestmark = pytest.mark.gpu
 
-TOL_BF16_F64 = 2e-2
+TOL_BF16_F64 = 7e-3  # 3x the 2.4e-3 measured on the BF16 3-chain
 TOL_F64_ORACLE = 1e-12
 
 
@@ -333,6 +333,6 @@ def test_serving_trace_1000_requests(gpu):
         if st["dtype"] == fg.DTYPE_INT8:
             assert np.array_equal(got, O.chain_int8(Bh, Ah, O.LAYOUT_MATH)), f"request {i}"
         elif st["dtype"] == fg.DTYPE_BF16:
-            assert O.rel_fro(got, O.chain_bf16(Bh, Ah, O.LAYOUT_MATH)) <= TOL_BF16, f"request {i}"
+            assert_rel(got, O.chain_bf16(Bh, Ah, O.LAYOUT_MATH), TOL_BF16, f"request {i}")
         else:
-            assert O.rel_fro(got, O.chain_f32(Bh, Ah)) <= TOL_F32, f"request {i}"
+            assert_rel(got, O.chain_f32(Bh, Ah), TOL_F32, f"request {i}")

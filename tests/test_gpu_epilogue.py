@@ -2,7 +2,7 @@
 ReLU) on the fused and unfused chain paths.
 
 Bars: INT8 bit-exact against the oracle (chain_int8_epi); FP32 relative Frobenius
-<= 1e-5 and BF16 <= 5e-3 against chain_float_epi; all extensions unset gives the
+<= 1e-5 and BF16 <= 5e-4 against chain_float_epi; all extensions unset gives the
 plain chain's bits.
 """
 import numpy as np
@@ -11,7 +11,7 @@ import pytest
 import flashgemm_amd as fg
 import oracle as O
 
-from test_gpu_parity import TOL_BF16, TOL_F32, dev, empty
+from test_gpu_parity import assert_rel, TOL_BF16, TOL_F32, dev, empty
 
 pytestmark = pytest.mark.gpu
 
@@ -103,7 +103,7 @@ def test_float_epilogue(gpu, dtype, fuse):
     B, As, epi = float_case(rng, dtype, 256, 768, [192, 256, 96])
     got = run(dtype, B, As, epi, fuse=fuse)
     want = O.chain_float_epi(B, As, epi, bf16=dtype == fg.DTYPE_BF16)
-    assert O.rel_fro(got, want) <= (TOL_F32 if dtype == fg.DTYPE_F32 else TOL_BF16)
+    assert_rel(got, want, (TOL_F32 if dtype == fg.DTYPE_F32 else TOL_BF16))
 
 
 def test_conv_chain_with_epilogue(gpu):

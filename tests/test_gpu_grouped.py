@@ -7,7 +7,7 @@ import pytest
 
 import flashgemm_amd as fg
 import oracle as O
-from test_gpu_parity import TOL_BF16, TOL_F32, dev, empty, rand_chain
+from test_gpu_parity import assert_rel, TOL_BF16, TOL_F32, dev, empty, rand_chain
 
 pytestmark = pytest.mark.gpu
 
@@ -39,9 +39,9 @@ def check_against_oracle(q, got):
     if q["dtype"] == fg.DTYPE_INT8:
         assert np.array_equal(got, O.chain_int8(q["B"], q["As"], O.LAYOUT_MATH))
     elif q["dtype"] == fg.DTYPE_BF16:
-        assert O.rel_fro(got, O.chain_bf16(q["B"], q["As"], O.LAYOUT_MATH)) <= TOL_BF16
+        assert_rel(got, O.chain_bf16(q["B"], q["As"], O.LAYOUT_MATH), TOL_BF16)
     else:
-        assert O.rel_fro(got, O.chain_f32(q["B"], q["As"])) <= TOL_F32
+        assert_rel(got, O.chain_f32(q["B"], q["As"]), TOL_F32)
 
 
 def single_launch(q):
@@ -106,7 +106,7 @@ def test_grouped_oversized_request_runs_unfused(gpu):
     assert not fg.grouped_fits(req)
     fg.grouped_chain([req])
     torch.cuda.synchronize()
-    assert O.rel_fro(C.cpu().numpy(), O.chain_f32(B, As)) <= TOL_F32
+    assert_rel(C.cpu().numpy(), O.chain_f32(B, As), TOL_F32)
 
 
 def test_graph_replays_match_eager(gpu):

@@ -7,9 +7,10 @@ Bars, stated per test:
   * FP32 (FP32 MFMA): relative Frobenius error <= 1e-5 against the reference /
     oracle and <= 1e-5 against FP64; plus the reference's own singlegemm check
     |C - C_ref| <= 1e-3 (singlegemm/FP32_ID23.c:125-128);
-  * BF16 chains: relative Frobenius error <= 5e-3 against the reference / oracle
-    (bf16 rounding of intermediates under a different fp32 summation order);
-  * FP32 split into bf16 hi/lo (3 products): relative Frobenius <= 1e-4 vs FP64.
+  * BF16 chains: relative Frobenius error <= 5e-4 against the reference / oracle
+    (bf16 rounding of intermediates under a different fp32 summation order; on
+    exact-arithmetic inputs the BF16 chains are bit-exact, test_gpu_exact.py);
+  * FP32 split into bf16 hi/lo (3 products): relative Frobenius <= 3e-5 vs FP64.
 """
 import numpy as np
 import pytest
@@ -21,9 +22,19 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
+# Bars: about 3x (FP32 5x) the largest error measured over the GPU suite (BF16 1.6e-4,
+# split-bf16 x3 8.7e-6 on the precision study, FP32 2.0e-6); the measured value is
+# printed by every check (assert_rel).
 TOL_F32 = 1e-5
-TOL_BF16 = 5e-3
-TOL_X3 = 1e-4
+TOL_BF16 = 5e-4
+TOL_X3 = 3e-5
+
+
+def assert_rel(got, want, tol, what=""):
+    """Relative Frobenius error of got against want, printed, then held to tol."""
+    err = O.rel_fro(got, want)
+    print(f"  rel err {what}: {err:.3e} (bar {tol:.0e})", flush=True)
+    assert err <= tol, f"rel err {what} {err:.3e} > {tol:.0e}"
 
 
 def dev(x):
@@ -81,8 +92,8 @@ def test_sgemm_fp32_mfma_vs_oracle(gpu, M, N, K):
     fg.sgemm(C, dev(A), dev(B), M, N, K)
     torch.cuda.synchronize()
     got = C.cpu().numpy()
-    assert O.rel_fro(got, O.sgemm(A, B)) <= TOL_F32
-    assert O.rel_fro(got, A.astype(np.float64) @ B.astype(np.float64)) <= TOL_F32
+    assert_rel(got, O.sgemm(A, B), TOL_F32)
+    assert_rel(got, A.astype(np.float64) @ B.astype(np.float64), TOL_F32)
 
 
 @pytest.mark.parametrize("f32_mode", [fg.F32_MFMA, fg.F32_BF16X3])
@@ -98,7 +109,7 @@ def test_sgemm_large_tile_vs_f64(gpu, f32_mode):
     fg.sgemm(C, dev(A), dev(B), M, N, K, f32_mode=f32_mode)
     torch.cuda.synchronize()
     tol = TOL_F32 if f32_mode == fg.F32_MFMA else TOL_X3
-    assert O.rel_fro(C.cpu().numpy(), A.astype(np.float64) @ B.astype(np.float64)) <= tol
+    assert_rel(C.cpu().numpy(), A.astype(np.float64) @ B.astype(np.float64), tol)
 
 
 @pytest.mark.parametrize("dtype", [fg.DTYPE_INT8, fg.DTYPE_BF16])
@@ -111,7 +122,7 @@ def test_unfused_chain_large_tile(gpu, dtype):
     if dtype == fg.DTYPE_INT8:
         assert np.array_equal(got, O.chain_int8(B, As, O.LAYOUT_MATH))
     else:
-        assert O.rel_fro(got, O.chain_bf16(B, As, O.LAYOUT_MATH)) <= TOL_BF16
+        assert_rel(got, O.chain_bf16(B, As, O.LAYOUT_MATH), TOL_BF16)
 
 
 @pytest.mark.parametrize("M,N,K,splits", [(64, 64, 4096, 8), (200, 40, 3000, 5), (64, 64, 65536, 0)])
@@ -129,7 +140,7 @@ def test_sgemm_split_k_deterministic(gpu, M, N, K, splits):
         torch.cuda.synchronize()
         outs.append(C.cpu().numpy())
     assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))  # fixed-order reduction
-    assert O.rel_fro(outs[0], A.astype(np.float64) @ B.astype(np.float64)) <= TOL_F32
+    assert_rel(outs[0], A.astype(np.float64) @ B.astype(np.float64), TOL_F32)
 
 
 def test_sgemm_split_bf16x3_vs_f64(gpu):
@@ -162,7 +173,7 @@ def test_sgemm_golden_through_reference_entry_point(gpu, name):
     fg.LibShalom_set_thread_nums(d["threads"])
     fg.LibShalom_sgemm_mp(0, 0, C, A, B, M, N, K)
     assert np.max(np.abs(C - d["C"])) <= 1e-3  # the reference's own check
-    assert O.rel_fro(C, d["C"]) <= TOL_F32
+    assert_rel(C, d["C"], TOL_F32)
 
 
 # ------------------------------------------------------------------ chains --
@@ -188,9 +199,9 @@ def test_chain_golden_through_reference_entry_point(gpu, name):
     if d["dtype"] == "int8":
         assert np.array_equal(C, d["C"]), f"{(C != d['C']).sum()} of {C.size} differ"
     elif d["dtype"] == "bf16":
-        assert O.rel_fro(C, d["C"]) <= TOL_BF16
+        assert_rel(C, d["C"], TOL_BF16)
     else:
-        assert O.rel_fro(C, d["C"]) <= TOL_F32
+        assert_rel(C, d["C"], TOL_F32)
 
 
 def expect_unfusable(dtype, K1, N, Ms, fuse):
@@ -255,7 +266,7 @@ def test_f32_chain_every_tile_width(gpu, tile_n, f32_mode):
         return
     got = run_chain(fg.DTYPE_F32, B, As, fuse=fg.FUSE_ON, tile_n=tile_n, f32_mode=f32_mode)
     tol = TOL_F32 if f32_mode == fg.F32_MFMA else TOL_X3
-    assert O.rel_fro(got, O.chain_f32(B, As)) <= tol
+    assert_rel(got, O.chain_f32(B, As), tol)
 
 
 @pytest.mark.parametrize("K1,N,Ms", CHAIN_SHAPES)
@@ -267,7 +278,7 @@ def test_bf16_chain_math_layout(gpu, K1, N, Ms, fuse):
     rng = np.random.default_rng(K1 * 3 + N)
     B, As = rand_chain(rng, fg.DTYPE_BF16, K1, N, Ms)
     got = run_chain(fg.DTYPE_BF16, B, As, fuse=fuse)
-    assert O.rel_fro(got, O.chain_bf16(B, As, O.LAYOUT_MATH)) <= TOL_BF16
+    assert_rel(got, O.chain_bf16(B, As, O.LAYOUT_MATH), TOL_BF16)
 
 
 @pytest.mark.parametrize("K1,N,Ms", CHAIN_SHAPES)
@@ -278,7 +289,7 @@ def test_f32_chain(gpu, K1, N, Ms, fuse):
     rng = np.random.default_rng(K1 * 5 + N)
     B, As = rand_chain(rng, fg.DTYPE_F32, K1, N, Ms)
     got = run_chain(fg.DTYPE_F32, B, As, fuse=fuse)
-    assert O.rel_fro(got, O.chain_f32(B, As)) <= TOL_F32
+    assert_rel(got, O.chain_f32(B, As), TOL_F32)
 
 
 @pytest.mark.parametrize("fuse", [fg.FUSE_ON, fg.FUSE_OFF])
@@ -287,7 +298,7 @@ def test_f32_chain_split_bf16x3(gpu, fuse):
     B, As = rand_chain(rng, fg.DTYPE_F32, 256, 512, [256, 256])
     got = run_chain(fg.DTYPE_F32, B, As, fuse=fuse, f32_mode=fg.F32_BF16X3)
     ref = O.chain_f64(B.astype(np.float64), [a.astype(np.float64) for a in As])
-    assert O.rel_fro(got, ref) <= TOL_X3
+    assert_rel(got, ref, TOL_X3)
 
 
 @pytest.mark.parametrize("dtype,f32_mode", [(fg.DTYPE_INT8, fg.F32_MFMA), (fg.DTYPE_BF16, fg.F32_MFMA),
@@ -350,7 +361,7 @@ def test_f32_row_major_weights_same_bits(gpu, K1, N, Ms, tile_n):
     torch.cuda.synchronize()
     want = C.cpu().numpy()
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"{(got != want).sum()} of {got.size} differ"
-    assert O.rel_fro(got, O.chain_f32(B, As)) <= TOL_F32
+    assert_rel(got, O.chain_f32(B, As), TOL_F32)
 
 
 @pytest.mark.parametrize("dtype,K1,Ms", [(fg.DTYPE_F32, 64, [128, 64, 96]), (fg.DTYPE_BF16, 128, [256, 64])])
@@ -369,9 +380,9 @@ def test_wide_tile_arena_whole_chunks(gpu, dtype, K1, Ms):
     assert e.value.status == fg.ERR_UNSUPPORTED
     got = run_chain(dtype, B, As)
     if dtype == fg.DTYPE_F32:
-        assert O.rel_fro(got, O.chain_f32(B, As)) <= TOL_F32
+        assert_rel(got, O.chain_f32(B, As), TOL_F32)
     else:
-        assert O.rel_fro(got, O.chain_bf16(B, As, O.LAYOUT_MATH)) <= TOL_BF16
+        assert_rel(got, O.chain_bf16(B, As, O.LAYOUT_MATH), TOL_BF16)
 
 
 @pytest.mark.parametrize("threads,N,Ms", [(1, 96, [24, 36, 12]), (4, 160, [12, 24, 36]), (2, 64, [36, 48, 24, 12]),
@@ -388,7 +399,7 @@ def test_bf16_reference_layout_device_api(gpu, threads, N, Ms):
     rng = np.random.default_rng(threads * 10 + N)
     B, As = rand_chain(rng, fg.DTYPE_BF16, 64, N, Ms)
     got = run_chain(fg.DTYPE_BF16, B, As, layout=fg.LAYOUT_REFERENCE, ref_threads=threads)
-    assert O.rel_fro(got, O.chain_bf16(B, As, O.LAYOUT_REFERENCE, threads)) <= TOL_BF16
+    assert_rel(got, O.chain_bf16(B, As, O.LAYOUT_REFERENCE, threads), TOL_BF16)
 
 
 def test_reference_layout_rejects_undefined_panels(gpu):

@@ -4,7 +4,7 @@
 Every output element sees the same fragments in the same k order whichever wave
 computes it, so the bar is bit-identity with the 32-column tile (every wave on all
 columns) for all kinds, plus the usual bars against the oracle (INT8 bit-exact,
-FP32 relative Frobenius <= 1e-5, BF16 <= 5e-3).
+FP32 relative Frobenius <= 1e-5, BF16 <= 5e-4).
 """
 import numpy as np
 import pytest
@@ -12,7 +12,7 @@ import pytest
 import flashgemm_amd as fg
 import oracle as O
 
-from test_gpu_parity import TOL_BF16, TOL_F32, rand_chain, run_chain
+from test_gpu_parity import assert_rel, TOL_BF16, TOL_F32, rand_chain, run_chain
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +50,7 @@ def test_wide_tile(gpu, dtype, tile_n, name):
     if dtype == fg.DTYPE_INT8:
         assert np.array_equal(got, want)
     else:
-        assert O.rel_fro(got, want) <= (TOL_F32 if dtype == fg.DTYPE_F32 else TOL_BF16)
+        assert_rel(got, want, (TOL_F32 if dtype == fg.DTYPE_F32 else TOL_BF16))
 
 
 def test_wide_tile_x3(gpu):

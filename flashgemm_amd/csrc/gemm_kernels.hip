@@ -533,24 +533,34 @@ __global__ __launch_bounds__(256) void colsum_i8_kernel(const int8_t *__restrict
 
 // Split-K reduction: a workgroup owns RE consecutive elements; RL lanes per element
 // each sum the slabs z = l, l + RL, ... in order, and lane sums are combined in
-// lane order through LDS, so the result does not depend on scheduling. With
-// has_ex it also applies the epilogue extensions (epi_i8 / epi_f, row m = e / N).
+// lane order through LDS, so the result does not depend on scheduling (nor on RE,
+// which only sets how many elements a workgroup takes: 64, or 16 for small outputs
+// with many slabs, so those spread over more CUs). A lane's loads are issued four
+// slabs at a time and added in slab order. With has_ex it also applies the epilogue
+// extensions (epi_i8 / epi_f, row m = e / N).
 constexpr int RE = 64;
-template <int KIND, int EPI, int RL>
-__global__ __launch_bounds__(RE * RL) void splitk_reduce_kernel(const void *__restrict__ P, int splits, long M,
-                                                                 long N, const int *__restrict__ colsum,
-                                                                 void *__restrict__ C, EpiOne ex, int has_ex) {
+template <int KIND, int EPI, int RL, int RE_ = RE>
+__global__ __launch_bounds__(RE_ * RL) void splitk_reduce_kernel(const void *__restrict__ P, int splits, long M,
+                                                                  long N, const int *__restrict__ colsum,
+                                                                  void *__restrict__ C, EpiOne ex, int has_ex) {
   using Sum = typename std::conditional<KIND == K_I8, unsigned, float>::type;
-  __shared__ Sum part[RL][RE];
-  const int x = threadIdx.x % RE, l = threadIdx.x / RE;
+  __shared__ Sum part[RL][RE_];
+  const int x = threadIdx.x % RE_, l = threadIdx.x / RE_;
   const size_t MN = (size_t)M * N;
-  const size_t e = (size_t)blockIdx.x * RE + x;
+  const size_t e = (size_t)blockIdx.x * RE_ + x;
   Sum acc = 0;
   if (e < MN) {
-    if (KIND == K_I8)
-      for (int z = l; z < splits; z += RL) acc += (unsigned)((const int *)P)[z * MN + e];
-    else
-      for (int z = l; z < splits; z += RL) acc += ((const float *)P)[z * MN + e];
+    const Sum *src = (const Sum *)P + e;
+    int z = l;
+    for (; z + 3 * RL < splits; z += 4 * RL) {
+      const Sum v0 = src[(size_t)z * MN], v1 = src[(size_t)(z + RL) * MN];
+      const Sum v2 = src[(size_t)(z + 2 * RL) * MN], v3 = src[(size_t)(z + 3 * RL) * MN];
+      acc += v0;
+      acc += v1;
+      acc += v2;
+      acc += v3;
+    }
+    for (; z < splits; z += RL) acc += src[(size_t)z * MN];
   }
   part[l][x] = acc;
   __syncthreads();
@@ -679,11 +689,22 @@ hipError_t launch_splitk_reduce(int kind, int epi, const void *P, int splits, lo
   dim3 grid((unsigned)((MN + RE - 1) / RE));
   // lanes per element: enough to keep every slab read in flight, at most 16
   const int rl = splits >= 16 ? 16 : splits >= 8 ? 8 : splits >= 4 ? 4 : splits >= 2 ? 2 : 1;
+  // many slabs of a small output: 16 elements per workgroup, so the reads spread over
+  // about one workgroup per CU instead of a quarter of them
+  const bool narrow = rl == 16 && MN <= (size_t)16 * 1024 * 4;
+  dim3 grid16((unsigned)((MN + 15) / 16));
 #define FG_RED1(K_, E_, L_) \
   hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_, L_>), grid, dim3(RE * L_), 0, s, P, splits, M, N, colsum, C, \
                      exv, has_ex)
+#define FG_RED16(K_, E_)                                                                                          \
+  hipLaunchKernelGGL((splitk_reduce_kernel<K_, E_, 16, 16>), grid16, dim3(256), 0, s, P, splits, M, N, colsum, C, \
+                     exv, has_ex)
 #define FG_RED(K_, E_)                          \
   do {                                          \
+    if (narrow) {                               \
+      FG_RED16(K_, E_);                         \
+      break;                                    \
+    }                                           \
     switch (rl) {                               \
       case 16: FG_RED1(K_, E_, 16); break;      \
       case 8: FG_RED1(K_, E_, 8); break;        \
@@ -701,6 +722,7 @@ hipError_t launch_splitk_reduce(int kind, int epi, const void *P, int splits, lo
   }
 #undef FG_RED
 #undef FG_RED1
+#undef FG_RED16
   return hipGetLastError();
 }
 
@@ -719,25 +741,126 @@ hipError_t launch_ref_permute(bool is_int8, const void *Z, void *X, long rows, c
 // ------------------------------------------------------------ skinny FP32 --
 // FP32 GEMMs with a small M or N (at most one 64-wide tile across it) are mostly
 // k-traffic: a 128 x 128 tile would waste half or three quarters of its MFMAs, so
-// they get a 64 x 64 tile per workgroup, split-K over the grid, no LDS staging.
-// Each of the 4 waves owns the whole 64 x 64 tile for k-groups g = wave, wave + 4,
-// ... (32 k per group) and loads its operands straight into MFMA fragment
-// registers, one group ahead: A[m][k..k+3] as 16 bytes per lane, B[k + s][n] as
-// coalesced dwords. The four wave tiles are summed through LDS in wave order.
+// they get a 64 x 64 tile per workgroup, split-K over the grid, no LDS. Each of the
+// 4 waves owns one 32 x 32 quadrant of the tile over the workgroup's whole k-slice,
+// so no wave waits for another and none exchanges data: it writes its quadrant of the
+// partial slab (or of C) itself. A wave streams its operands straight into MFMA
+// fragment registers, SK_PD groups of 32 k ahead of its MFMAs: A[m][k..k+3] as 16
+// bytes per lane, B[k + s][n] as coalesced dwords. Within a group the k order is
+// fixed and the even and odd MFMAs of a k-step go to two accumulators added at the
+// end, so results do not depend on timing. That is the kernel for short slices
+// (split-K-heavy shapes); for long slices skinny_tile_kernel below puts every wave on
+// the whole tile instead (twice the operand reuse), and its four wave tiles are summed
+// through LDS, an exchange that cost ~1.7 us of the 64 x 64 x 65536 GEMM.
 namespace {
 
 constexpr int SK_TILE = 64;
-constexpr int SK_GROUP = 32;  // k per wave group (4 MFMA k-steps of 8)
+constexpr int SK_GROUP = 32;  // k per group (4 MFMA k-steps of 8)
+constexpr int SK_QGROUPS = 8; // groups per slice of the quadrant kernel (256 k)
 
 struct SkFrag {
+  f32x4 a[4];     // [k-step t]: A[row][k + 8t + 4h .. + 3]
+  float b[4][4];  // [k-step t][s]: B[k + 8t + 4h + s][col]
+};
+
+// One group of a quadrant for lane (h, l32): A row `row`, B column `col`, k [k, k + 32)
+// clipped to ke. fast: the quadrant lies inside A and B, the whole group inside the
+// slice and the A rows are 16-byte aligned (K % 4 == 0).
+__device__ __forceinline__ void sk_load(const float *__restrict__ A, const float *__restrict__ B, long M, long N,
+                                        long K, long ke, long row, long col, long k, bool fast, int h, SkFrag &f) {
+  if (fast) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) f.a[t] = *(const f32x4 *)(A + row * K + k + 8 * t + 4 * h);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s) f.b[t][s] = B[(k + 8 * t + 4 * h + s) * N + col];
+    return;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const long kk = k + 8 * t + 4 * h + s;
+      f.a[t][s] = (row < M && kk < ke) ? A[row * K + kk] : 0.0f;
+      f.b[t][s] = (col < N && kk < ke) ? B[kk * N + col] : 0.0f;
+    }
+}
+
+template <int SK_PD>
+__global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict__ A, const float *__restrict__ B,
+                                                         float *__restrict__ C, long M, long N, long K, long kper,
+                                                         int tiles_n, int partial) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int h = lane >> 5, l32 = lane & 31;
+  // quadrant (wave >> 1, wave & 1) of the workgroup's 64 x 64 tile
+  const long m0 = (long)(blockIdx.x / tiles_n) * SK_TILE + 32 * (wave >> 1);
+  const long n0 = (long)(blockIdx.x % tiles_n) * SK_TILE + 32 * (wave & 1);
+  if (m0 >= M || n0 >= N) return;  // the quadrant lies outside C (no barrier follows)
+  const long kb = (long)blockIdx.y * kper;
+  const long ke = kb + kper < K ? kb + kper : K;
+  const long row = m0 + l32, col = n0 + l32;
+  const bool inside = m0 + 32 <= M && n0 + 32 <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0;
+  f32x16 acc, acc2;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    acc[r] = 0.0f;
+    acc2[r] = 0.0f;
+  }
+  // the slice holds at most SK_QGROUPS groups (kper == SK_QGROUPS * SK_GROUP); groups
+  // past the end of K (the last slice) are skipped with a uniform branch
+  const int ng = (int)((ke - kb + SK_GROUP - 1) / SK_GROUP);
+  auto load = [&](int g, SkFrag &f) {
+    const long k = kb + (long)g * SK_GROUP;
+    sk_load(A, B, M, N, K, ke, row, col, k, inside && k + SK_GROUP <= ke, h, f);
+  };
+  auto mma = [&](const SkFrag &f) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; s += 2) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(f.a[t][s], f.b[t][s], acc, 0, 0, 0);
+        acc2 = __builtin_amdgcn_mfma_f32_32x32x2f32(f.a[t][s + 1], f.b[t][s + 1], acc2, 0, 0, 0);
+      }
+  };
+  // ring of SK_PD + 1 fragment buffers, the whole slice unrolled at compile time so
+  // every buffer index is a constant (registers, not scratch) and the compiler sees
+  // the full load / MFMA schedule (a runtime-length loop measured 13.2 against 9.5 us)
+  SkFrag f[SK_PD + 1];
+#pragma unroll
+  for (int g = 0; g < SK_PD; ++g)
+    if (g < ng) load(g, f[g]);
+#pragma unroll
+  for (int g = 0; g < SK_QGROUPS; ++g) {
+    if (g >= ng) break;
+    if (g + SK_PD < ng) load(g + SK_PD, f[(g + SK_PD) % (SK_PD + 1)]);
+    mma(f[g % (SK_PD + 1)]);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] += acc2[r];
+  // lane holds rows 8(r/4) + 4h + (r%4) of column l32 of the quadrant
+  float *out = partial ? C + (size_t)blockIdx.y * M * N : C;
+  if (col >= N) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const long m = m0 + 8 * (r >> 2) + 4 * h + (r & 3);
+    if (m < M) out[m * N + col] = acc[r];
+  }
+}
+
+// Long slices: each of the 4 waves computes the whole 64 x 64 tile for the groups
+// g = wave, wave + 4, ... of the slice (so each loaded A fragment meets two B column
+// blocks and each B fragment two A row blocks), one group ahead; the four wave tiles
+// are summed through LDS in wave order.
+struct SkTileFrag {
   f32x4 a[4][2];     // [k-step][row block]: k = 8t + 4h .. 8t + 4h + 3
   float b[4][2][4];  // [k-step][column block][s]: B[8t + 4h + s][32j + l32]
 };
 
-__device__ __forceinline__ void sk_load(const float *__restrict__ A, const float *__restrict__ B, long M, long N,
-                                        long K, long ke, long m0, long n0, long k, bool interior, int h, int l32,
-                                        SkFrag &f) {
-  if (interior) {
+__device__ __forceinline__ void sk_tile_load(const float *__restrict__ A, const float *__restrict__ B, long M,
+                                             long N, long K, long ke, long m0, long n0, long k, bool fast, int h,
+                                             int l32, SkTileFrag &f) {
+  if (fast) {
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
 #pragma unroll
@@ -773,16 +896,16 @@ __device__ __forceinline__ void sk_load(const float *__restrict__ A, const float
   }
 }
 
-__global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict__ A, const float *__restrict__ B,
-                                                         float *__restrict__ C, long M, long N, long K, long kper,
-                                                         int tiles_n, int partial) {
+__global__ __launch_bounds__(256) void skinny_tile_kernel(const float *__restrict__ A, const float *__restrict__ B,
+                                                          float *__restrict__ C, long M, long N, long K, long kper,
+                                                          int tiles_n, int partial) {
   __shared__ float red[SK_TILE * SK_TILE];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int h = lane >> 5, l32 = lane & 31;
   const long m0 = (long)(blockIdx.x / tiles_n) * SK_TILE, n0 = (long)(blockIdx.x % tiles_n) * SK_TILE;
   const long kb = (long)blockIdx.y * kper;
   const long ke = kb + kper < K ? kb + kper : K;
-  const bool rows_in = m0 + SK_TILE <= M && n0 + SK_TILE <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0;
+  const bool inside = m0 + SK_TILE <= M && n0 + SK_TILE <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0;
   f32x16 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -790,10 +913,10 @@ __global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  SkFrag f[2];
+  SkTileFrag f[2];
   long k = kb + (long)wave * SK_GROUP;
-  if (k < ke) sk_load(A, B, M, N, K, ke, m0, n0, k, rows_in && k + SK_GROUP <= ke, h, l32, f[0]);
-  auto mma = [&](const SkFrag &g) {
+  if (k < ke) sk_tile_load(A, B, M, N, K, ke, m0, n0, k, inside && k + SK_GROUP <= ke, h, l32, f[0]);
+  auto mma = [&](const SkTileFrag &g) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -804,21 +927,19 @@ __global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict
           for (int i = 0; i < 2; ++i)
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(g.a[t][i][s], g.b[t][j][s], acc[i][j], 0, 0, 0);
   };
-  // two groups per iteration, buffers alternating
-  for (; k < ke; k += 8 * SK_GROUP) {
+  for (; k < ke; k += 8 * SK_GROUP) {  // two groups per iteration, buffers alternating
     const long k1 = k + 4 * SK_GROUP, k2 = k + 8 * SK_GROUP;
-    if (k1 < ke) sk_load(A, B, M, N, K, ke, m0, n0, k1, rows_in && k1 + SK_GROUP <= ke, h, l32, f[1]);
+    if (k1 < ke) sk_tile_load(A, B, M, N, K, ke, m0, n0, k1, inside && k1 + SK_GROUP <= ke, h, l32, f[1]);
     __builtin_amdgcn_sched_barrier(0);
     mma(f[0]);
     __builtin_amdgcn_sched_barrier(0);
     if (k1 >= ke) break;
-    if (k2 < ke) sk_load(A, B, M, N, K, ke, m0, n0, k2, rows_in && k2 + SK_GROUP <= ke, h, l32, f[0]);
+    if (k2 < ke) sk_tile_load(A, B, M, N, K, ke, m0, n0, k2, inside && k2 + SK_GROUP <= ke, h, l32, f[0]);
     __builtin_amdgcn_sched_barrier(0);
     mma(f[1]);
     __builtin_amdgcn_sched_barrier(0);
   }
-  // sum the wave tiles in wave order: lane holds rows 8(r/4) + 4h + (r%4) of
-  // column 32j + l32 in row block i
+  // lane holds rows 8(r/4) + 4h + (r%4) of column 32j + l32 in row block i
   for (int w = 0; w < 4; ++w) {
     if (wave == w) {
 #pragma unroll
@@ -851,6 +972,8 @@ int skinny_f32_splits(long M, long N, long K, long *kper_out) {
   long kper = (K + want - 1) / want;
   kper = (kper + 4 * SK_GROUP - 1) / (4 * SK_GROUP) * (4 * SK_GROUP);
   if (kper < 8 * SK_GROUP) kper = 8 * SK_GROUP;  // at least two groups per wave
+  // short slices: the quadrant kernel's fixed 256-k slice (more slabs, same work)
+  if (kper <= 16 * SK_GROUP) kper = SK_QGROUPS * SK_GROUP;
   *kper_out = kper;
   return (int)((K + kper - 1) / kper);
 }
@@ -858,8 +981,17 @@ int skinny_f32_splits(long M, long N, long K, long *kper_out) {
 hipError_t launch_skinny_f32(const float *A, const float *B, float *C, long M, long N, long K, long kper,
                              int splits, hipStream_t s) {
   const int tiles_m = (int)((M + SK_TILE - 1) / SK_TILE), tiles_n = (int)((N + SK_TILE - 1) / SK_TILE);
-  hipLaunchKernelGGL(skinny_f32_kernel, dim3((unsigned)(tiles_m * tiles_n), (unsigned)splits), dim3(256), 0, s, A, B,
-                     C, M, N, K, kper, tiles_n, splits > 1 ? 1 : 0);
+  const dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)splits);
+  const int partial = splits > 1;
+  // Short slices (the split-K-heavy shapes, e.g. 64 x 64 x 65536 with 256 slices of
+  // 256 k): one quadrant per wave, nothing exchanged (kernel 13.0 against 16.8 us for
+  // that shape, rocprofv3). Long slices (tall shapes such as 8192 x 64 x 8192, 4096 k
+  // per slice): every wave on the whole tile, which feeds each loaded fragment to two
+  // MFMAs instead of one (~100 against 140 us for that shape).
+  if (kper == SK_QGROUPS * SK_GROUP)
+    hipLaunchKernelGGL(skinny_f32_kernel<2>, grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
+  else
+    hipLaunchKernelGGL(skinny_tile_kernel, grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
   return hipGetLastError();
 }
 

@@ -122,6 +122,7 @@ def lib() -> ctypes.CDLL:
                                          ctypes.POINTER(ChainOpts)]
         L.fg_sgemm.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_t.argtypes = [i, i, vp, vp, vp, l, l, l, i, i, vp]
+        L.fg_sgemm_noalloc.argtypes = [vp, vp, vp, l, l, l, i, i, vp]
         L.fg_sgemm_host.argtypes = [vp, vp, vp, l, l, l, i]
         L.fg_f64_chain.argtypes = [vp, vp, l, i, vpp, lp, lp, vp]
         L.fg_to_f64.argtypes = [i, vp, vp, l, vp]
@@ -280,6 +281,15 @@ def sgemm_t(transa: int, transb: int, C, A, B, M: int, N: int, K: int, f32_mode:
     """C (M x N) = op(A) . op(B); transa: A stored K x M, transb: B stored N x K."""
     _check(lib().fg_sgemm_t(transa, transb, _addr(C), _addr(A), _addr(B), M, N, K, f32_mode, splits,
                             _stream(stream)))
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20  # FG_NOALLOC_SCRATCH_BYTES
+
+
+def sgemm_noalloc(C, A, B, M: int, N: int, K: int, f32_mode: int = F32_MFMA, splits: int = 0, stream=None) -> None:
+    """fg_sgemm_noalloc: sgemm with its split-K slabs in the library's static device array
+    (no allocation; calls overlapping on one GPU race)."""
+    _check(lib().fg_sgemm_noalloc(_addr(C), _addr(A), _addr(B), M, N, K, f32_mode, splits, _stream(stream)))
 
 
 def f64_chain(C, B, As, N: int | None = None, Ms=None, Ks=None, stream=None) -> None:

@@ -26,6 +26,7 @@ SOURCES = [
     "api.cpp",
     "serving.cpp",
     "autotune.cpp",
+    "noalloc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

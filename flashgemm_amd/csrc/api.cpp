@@ -343,16 +343,28 @@ int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K,
 // epilogue extensions (ex != nullptr) the GEMM always writes fp32 / int32 slabs and
 // the reduction pass applies the extensions while it forms the output.
 // trans (FP32 kinds): bit 0 A stored K x M, bit 1 B stored N x K.
+// ws: a caller-owned workspace of ws_bytes for the split-K slabs (FP32 and BF16
+// kinds), used instead of the stream's scratch so that nothing is allocated; a plan
+// that needs more than ws_bytes is refused with FG_ERR_INVALID.
 static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K, int splits,
-                    hipStream_t s, const EpiOne *ex = nullptr, int trans = 0) {
+                    hipStream_t s, const EpiOne *ex = nullptr, int trans = 0, void *ws = nullptr,
+                    size_t ws_bytes = 0) {
   int *colsum = nullptr;
   int rc;
+  auto partials = [&](size_t bytes, void **p) -> int {
+    if (!ws) return scratch(s, "partials", bytes, p);
+    if (bytes > ws_bytes)
+      return set_error(FG_ERR_INVALID, "split-K slabs need %zu bytes, the workspace holds %zu (M=%ld N=%ld K=%ld)",
+                       bytes, ws_bytes, M, N, K);
+    *p = ws;
+    return FG_OK;
+  };
   if (kind == K_F32 && epi != EPI_PARTIAL && splits <= 0 && trans == 0 && skinny_f32_shape(M, N)) {
     // small M or N: 64 x 64 tiles, split-K over the grid
     long kper = 0;
     const int sk = skinny_f32_splits(M, N, K, &kper);
     float *out = (float *)C;
-    if ((sk > 1 || ex) && (rc = scratch(s, "partials", (size_t)sk * M * N * 4, (void **)&out))) return rc;
+    if ((sk > 1 || ex) && (rc = partials((size_t)sk * M * N * 4, (void **)&out))) return rc;
     if ((rc = hip_status(launch_skinny_f32((const float *)A, (const float *)B, out, M, N, K, kper, sk, s),
                          "skinny gemm")))
       return rc;
@@ -369,7 +381,7 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
     if ((rc = hip_status(launch_colsum_i8((const int8_t *)B, K, N, colsum, s), "colsum"))) return rc;
   }
   void *part = nullptr;
-  if ((rc = scratch(s, "partials", (size_t)splits * M * N * 4, &part))) return rc;
+  if ((rc = partials((size_t)splits * M * N * 4, &part))) return rc;
   GemmArgs g{kind, EPI_PARTIAL, A, B, part, colsum, M, N, K, splits, trans};
   if ((rc = hip_status(launch_gemm(g, s), "gemm(split-K)"))) return rc;
   // the kernel may use fewer slabs than asked when K is short
@@ -538,6 +550,24 @@ int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, i
   int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
   return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream);
 }
+
+}  // extern "C"
+
+namespace fg {
+// fg_sgemm with the split-K slabs in the workspace ws (ws_bytes) instead of the
+// stream's scratch: the same plan and kernels, so the same bits, and no allocation.
+int sgemm_ws(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode, int splits, void *stream,
+             void *ws, size_t ws_bytes) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
+    return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
+  const int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, 0, ws, ws_bytes);
+}
+}  // namespace fg
+
+extern "C" {
 
 int fg_sgemm_t(int transa, int transb, float *C, const float *A, const float *B, long M, long N, long K,
                int f32_mode, int splits, void *stream) {

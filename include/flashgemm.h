@@ -144,6 +144,18 @@ int fg_sgemm_host(float *C, const float *A, const float *B, long M, long N, long
 int fg_sgemm_t(int transa, int transb, float *C, const float *A, const float *B, long M, long N, long K,
                int f32_mode, int splits, void *stream);
 
+/* fg_sgemm that allocates nothing. No counterpart in the reference (whose drivers
+ * allocate per call); for callers that cannot call hipMalloc: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation. Same plan, kernels
+ * and bits as fg_sgemm; its split-K slabs live in a static device array of
+ * FG_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of library
+ * scratch. A plan whose slabs (splits x M x N x 4 bytes) exceed that returns
+ * FG_ERR_INVALID. Every call on one GPU shares the array: calls that overlap on one
+ * GPU (other streams, other threads) race. */
+#define FG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int fg_sgemm_noalloc(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode, int splits,
+                     void *stream);
+
 /* Weights in the fused kernels' packed layout: MFMA fragment order, zero padded,
  * INT8 bytes pre-biased to signed, FP32 for FG_F32_BF16X3 pre-split into bf16 hi/lo.
  * The reference packs A inside every call (multigemm/src/PACK_x86.h:616-647,

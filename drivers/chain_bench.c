@@ -1,7 +1,7 @@
 /* Chain benchmark driver: the structure of the reference's multigemm drivers
  * (multigemm/test/test_{f32,bf16,int8}{,_3}.c) on libflashgemm.
  *
- *   chain_bench <f32|bf16|int8> <1|3|conv> [threads]
+ *   chain_bench <f32|bf16|int8> <1|3|conv|x3> [threads]
  *
  * Mode 1 runs each of the reference's 20 conv-layer GEMM shapes as a one-GEMM
  * chain (multigemm/test/test_f32.c:14-24); mode 3 runs its 10 three-GEMM chains
@@ -10,7 +10,8 @@
  * Mode conv runs the same 10 chains as the convolutions they stand for: GEMM 1 is
  * a k x k convolution (k = 3 when 9 divides K1, else 5; pad k/2, stride 1) of a
  * K1/k^2-channel sqrt(N) x sqrt(N) image, through fg_chain_conv_host, which
- * gathers the patches on the GPU instead of taking a formed B.
+ * gathers the patches on the GPU instead of taking a formed B. Mode x3 runs the
+ * benchmark's 3-GEMM chain (N = 16384, 512 -> 512 -> 1024 -> 512).
  * `threads` sets flashgemm_thread_num (the reference layout's panel count). */
 #include "drivers_common.h"
 #include "flashgemm.h"
@@ -120,6 +121,13 @@ int main(int argc, char **argv)
     const int conv = argc > 2 && strcmp(argv[2], "conv") == 0;
     int mode = argc > 2 && !conv ? atoi(argv[2]) : 3;
     flashgemm_set_thread_num(argc > 3 ? atoi(argv[3]) : 1);
+    if (argc > 2 && strcmp(argv[2], "x3") == 0) {
+        /* the benchmark's three-GEMM chain X(16384 x 512).W1(512 x 512).W2(512 x 1024).W3(1024 x 512),
+           reference orientation: N = 16384, 512 -> 512 -> 1024 -> 512 */
+        long M[3] = {512, 1024, 512}, K[3] = {512, 512, 1024};
+        run(dt, 16384, 3, M, K);
+        return 0;
+    }
     for (int j = 0; j < 10; j++) {
         if (conv) {
             long M[3] = {CH1[3 * j], CH2[3 * j], CH3[3 * j]};

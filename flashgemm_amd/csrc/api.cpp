@@ -295,14 +295,26 @@ static int ref_panels(long N, int threads, RefPanels *p) {
 // Decision order: explicit options, then the autotune table, then the heuristics
 // (widest tile that fits and still fills the CUs; split-K for skinny GEMMs).
 // epi: the chain has epilogue extensions (their per-row tables take LDS too).
+// The reference layout runs fused (the epilogues write the reference's permuted
+// intermediates, *refl = 1) when every reference panel is a 32-aligned block and every
+// intermediate M_i a multiple of 32 (chain_refl_ok), without epilogue extensions or an
+// image B (img); otherwise it runs unfused with a gather after each GEMM.
 static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
-                      int *fused, int *tile_n, int *splits, bool epi = false) {
+                      int *fused, int *tile_n, int *splits, bool epi = false, bool img = false,
+                      int *refl_out = nullptr) {
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
   int kind = kind_of(dtype, o.f32_mode);
-  bool ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
+  const bool any_ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
+  bool refl = false;
+  if (any_ref && !epi && !img) {
+    RefPanels p;
+    refl = ref_panels(N, o.ref_threads, &p) == FG_OK && chain_refl_ok(kind, p, num_gemm, M);
+  }
+  if (refl_out) *refl_out = refl;
+  const bool ref = any_ref && !refl;  // the reference layout through the unfused gather path
   *splits = o.splits;
-  if (!ref && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
+  if (!any_ref && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
     int f, t, sp;
     // an entry that cannot run this exact shape (a tile width the kind lacks, or an
     // arena beyond LDS) is ignored: the heuristics below decide instead
@@ -494,10 +506,12 @@ static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, c
 
   int fused = 0, tn = 0;
   int splits = 0;
-  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr))) return rc;
+  int refl = 0;
+  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr, false, &refl))) return rc;
   if (fused) {
     ChainArgs c = chain_args(kind, B, C, N, num_gemm, A, M, K, tn);
     c.epi = ep;
+    c.refl = refl;
     ChainParams P;
     make_chain_params(c, &P);
     if (chain_fused_aligned(P)) return run_fused(c, o.a_packed, s);
@@ -668,7 +682,8 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
   // The gathering kernel loads single elements, so the fused path takes any N: plan
   // with N rounded up to whole 4-column groups.
   int fused = 0, tn = 0, splits = 0;
-  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr))) return rc;
+  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr, true)))
+    return rc;
   if (fused) {
     ChainArgs c = chain_args(kind, X, C, N, num_gemm, A, M, K, tn);
     c.img = &ig;

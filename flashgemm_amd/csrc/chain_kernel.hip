@@ -435,6 +435,69 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
   }
 }
 
+// Intermediate epilogue in the reference's layout (ChainParams::refl). The reference's
+// GEMM i+1 reads, inside each 32-column panel, X[k][c] = Z[src_row][src_col]
+// (ref_src: INT8 (k&~3) + (c>>3), 4(c&7) + (k&3); BF16 (k&~1) + (c>>4), 2(c&15) + (k&1)),
+// a bijection within 4 (INT8) or 2 (BF16) rows of the panel. When every panel is a
+// 32-aligned block of the tile, GEMM i's epilogue applies the inverse while it writes
+// the arena: Z[m][j] goes to
+//   INT8: X[4(m>>2) + (j&3)][8(m&3) + (j>>2)]     BF16: X[2(m>>1) + (j&1)][16(m&1) + (j>>1)]
+// of its panel. Lane (h, l32) holds rows m = 8q + 4h + e (e = 0..3) of panel column
+// j = l32, so its four values land in one X row (INT8) or two (BF16), in four panel
+// columns: byte / halfword LDS stores. Every M_i is a multiple of 32 on this path (no
+// padded rows inside a block). INT8 also adds each value to the column sum of the X
+// column it lands in (the next GEMM's u8 bias).
+template <int KIND, int TN, int TNW>
+__device__ __forceinline__ void store_rows_refl(const ChainParams &P, int gi, long rb0_l, uint4 *arena,
+                                                const typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNW], int *csum,
+                                                int cb0) {
+  using T = CT<KIND>;
+  constexpr int RP = T::RP;
+  const int S = P.S;
+  const int lane = fresh_tid() & 63;
+  const int h = lane >> 5, l32 = lane & 31;
+  const int Mp = (int)P.M[gi];
+  const int rb0 = (int)rb0_l;
+  int cs[TNW][4];
+#pragma unroll
+  for (int j = 0; j < TNW; ++j)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cs[j][e] = 0;
+#pragma unroll
+  for (int i = 0; i < RP; ++i) {
+    const int rbase = rb0 + i * 32;
+    if (rbase >= Mp) continue;
+    int rs = P.ybase[gi] + rbase;  // arena row of the block (a multiple of 32; no wrap inside)
+    if (rs >= S) rs -= S;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+      for (int j = 0; j < TNW; ++j) {
+        const int col0 = (cb0 + j) * 32;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          if (KIND == K_I8) {
+            const int k = rs + 8 * q + 4 * h + (l32 & 3);
+            const int c = col0 + 8 * e + (l32 >> 2);
+            const int8_t v = sat_i8((int)acc[i][j][4 * q + e]);
+            ((int8_t *)(arena + (k / T::KG) * TN + c))[k % T::KG] = v;
+            cs[j][e] += v;
+          } else {
+            const int k = rs + 8 * q + 4 * h + 2 * (e >> 1) + (l32 & 1);
+            const int c = col0 + 16 * (e & 1) + (l32 >> 1);
+            ((uint16_t *)(arena + (k / T::KG) * TN + c))[k % T::KG] = bf16_dev((float)acc[i][j][4 * q + e]);
+          }
+        }
+      }
+    }
+  }
+  if (KIND == K_I8)
+#pragma unroll
+    for (int j = 0; j < TNW; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) atomicAdd(&csum[(cb0 + j) * 32 + 8 * e + (l32 >> 2)], cs[j][e]);
+}
+
 // Epilogue of one pass: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32 of
 // row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena (INT8
 // also adds the s8 column sums of what it writes into csum: the next GEMM's bias).
@@ -501,6 +564,12 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
       }
     }
     return;
+  }
+  if constexpr (KIND == K_I8 || KIND == K_BF16) {
+    if (P.refl) {
+      store_rows_refl<KIND, TN, TNW>(P, gi, rb0, arena, acc, csum, cb0);
+      return;
+    }
   }
   // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q. Padded rows
   // (m >= M) are stored as zero: they are the next GEMM's k-padding. A 32-row block
@@ -951,6 +1020,8 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   ChainParams P{};
   make_chain_params(c, &P);
   if (!c.img && !chain_fused_aligned(P)) return hipErrorInvalidValue;
+  // reference-layout intermediates: INT8 / BF16 matrix chains with the plain epilogues
+  if (c.refl && (c.img || c.epi || (KIND != K_I8 && KIND != K_BF16))) return hipErrorInvalidValue;
   constexpr int TN = TNB * 32;
   const size_t lds = c.epi ? (size_t)P.epi_off + chain_epi_lds_bytes(c.M, c.num_gemm) : arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
@@ -1122,6 +1193,15 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
   P->vecB = (c.N % 4 == 0) && (((uintptr_t)c.B & (4 * e - 1)) == 0);
   if (c.img) P->img = *c.img;
   if (c.epi) P->epi = *c.epi;
+  P->refl = c.refl;
+}
+
+bool chain_refl_ok(int kind, const RefPanels &p, int num_gemm, const long *M) {
+  if (kind != K_I8 && kind != K_BF16) return false;
+  if (p.nb % 32 != 0 || p.ne % 32 != 0) return false;
+  for (int i = 0; i + 1 < num_gemm; ++i)
+    if (M[i] % 32 != 0) return false;
+  return true;
 }
 
 bool chain_fused_shape_ok(int kind, long N, int num_gemm, const long *K) {

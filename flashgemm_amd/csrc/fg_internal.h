@@ -233,7 +233,12 @@ struct ChainArgs {
   const ImgGeom *img;       // nullptr: B is a matrix; else B is gathered from the image
   const EpiParams *epi;     // nullptr: the reference's epilogues; else per-GEMM extensions
   int raw_a;                // A[i] are the row-major FP32 weights, read unpacked (chain_raw_weights_ok)
+  int refl;                 // INT8 / BF16: intermediates in the reference's layout (chain_refl_ok)
 };
+// The fused kernels can write INT8 / BF16 intermediates in the reference's layout
+// when every reference panel is a 32-aligned block (every thread's column range a
+// multiple of 32, see RefPanels) and every intermediate M_i is a multiple of 32.
+bool chain_refl_ok(int kind, const RefPanels &p, int num_gemm, const long *M);
 // FP32 chains whose weights need no padding (every M_i % 32 == 0, every K_i a whole
 // number of 64-k chunks, 16-byte aligned) on a matrix B with the plain epilogues: the
 // fused kernel reads A_i row-major in fragment order and the per-call pack is skipped.
@@ -258,6 +263,7 @@ struct ChainParams {
   ImgGeom img;              // geometry when B is gathered from an image (chain_kernel IMG)
   EpiParams epi;            // epilogue extensions (chain_kernel EPX)
   int epi_off;              // EPX: LDS byte offset of the per-row bias and scale tables
+  int refl;                 // intermediates in the reference's layout (store_rows_refl)
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);

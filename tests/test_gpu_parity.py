@@ -402,6 +402,38 @@ def test_bf16_reference_layout_device_api(gpu, threads, N, Ms):
     assert_rel(got, O.chain_bf16(B, As, O.LAYOUT_REFERENCE, threads), TOL_BF16)
 
 
+@pytest.mark.parametrize("dtype", [fg.DTYPE_INT8, fg.DTYPE_BF16])
+@pytest.mark.parametrize("tile_n", [32, 64, 128, 256])
+@pytest.mark.parametrize("threads", [1, 2, 4, 3])
+def test_reference_layout_fused(gpu, dtype, tile_n, threads):
+    """The reference's intermediate layout on the fused kernel: with 32-aligned panels
+    (1, 2, 4 threads over N = 512) and intermediates of whole 32-row blocks, the fused
+    chain writes the permuted intermediates itself; INT8 equals the unfused gather path
+    and the oracle bit for bit, BF16 the oracle within TOL_BF16. With 3 threads the
+    panels are 176 wide (a 16-wide edge), so only the unfused path applies."""
+    print(f"reference layout fused dtype={dtype} tile={tile_n} threads={threads}", flush=True)
+    rng = np.random.default_rng(tile_n * 10 + threads + dtype)
+    K1, N, Ms = 64, 512, [96, 64, 128]
+    Ks = [K1] + Ms[:-1]
+    B, As = rand_chain(rng, dtype, K1, N, Ms, full_int8=True)
+    kw = dict(layout=fg.LAYOUT_REFERENCE, ref_threads=threads)
+    fused, _ = fg.chain_plan(dtype, N, Ms, Ks, fg.opts(tile_n=tile_n, **kw))
+    want = (O.chain_int8 if dtype == fg.DTYPE_INT8 else O.chain_bf16)(B, As, O.LAYOUT_REFERENCE, threads)
+    if threads == 3 or not fused:
+        assert threads == 3 or (dtype == fg.DTYPE_BF16 and tile_n == 256)
+        with pytest.raises(fg.FlashgemmError) as e:
+            run_chain(dtype, B, As, fuse=fg.FUSE_ON, tile_n=tile_n, **kw)
+        assert e.value.status == fg.ERR_UNSUPPORTED
+        got = run_chain(dtype, B, As, **kw)
+    else:
+        got = run_chain(dtype, B, As, fuse=fg.FUSE_ON, tile_n=tile_n, **kw)
+    if dtype == fg.DTYPE_INT8:
+        assert np.array_equal(got, want), f"{(got != want).sum()} of {got.size} differ"
+        assert np.array_equal(got, run_chain(dtype, B, As, fuse=fg.FUSE_OFF, **kw))
+    else:
+        assert_rel(got, want, TOL_BF16)
+
+
 def test_reference_layout_rejects_undefined_panels(gpu):
     rng = np.random.default_rng(0)
     B, As = rand_chain(rng, fg.DTYPE_INT8, 64, 40, [24])  # N=40 -> a panel of width 8

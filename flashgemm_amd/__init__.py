@@ -141,6 +141,9 @@ def lib() -> ctypes.CDLL:
         L.fg_pack_weight.argtypes = [i, i, vp, l, l, vp, vp]
         L.fg_grouped_chain.argtypes = [i, ctypes.POINTER(ChainReq), vp]
         L.fg_grouped_fits.argtypes = [ctypes.POINTER(ChainReq)]
+        L.fg_grouped_tile.argtypes = [ctypes.POINTER(ChainReq)]
+        L.fg_chain_plan_segments.argtypes = [i, l, i, lp, lp, ctypes.POINTER(ChainOpts), ctypes.POINTER(i),
+                                             ctypes.POINTER(i), ctypes.POINTER(i)]
         L.fg_grouped_graph_create.argtypes = [i, i, ctypes.POINTER(vp)]
         L.fg_grouped_graph_launch.argtypes = [vp, i, ctypes.POINTER(ChainReq), vp]
         L.fg_grouped_graph_destroy.argtypes = [vp]
@@ -232,6 +235,16 @@ def chain_plan(dtype: int, N: int, Ms, Ks, o: ChainOpts | None = None):
     _check(lib().fg_chain_plan(dtype, N, n, (ctypes.c_long * n)(*Ms), (ctypes.c_long * n)(*Ks),
                                ctypes.byref(o) if o is not None else None, ctypes.byref(fused), ctypes.byref(tn)))
     return bool(fused.value), tn.value
+
+
+def chain_plan_segments(dtype: int, N: int, Ms, Ks, o: ChainOpts | None = None):
+    """The planner's segments: a list of (last GEMM index, tile width or 0 = unfused GEMM)."""
+    n = len(Ms)
+    ns = ctypes.c_int()
+    ends, tiles = (ctypes.c_int * n)(), (ctypes.c_int * n)()
+    _check(lib().fg_chain_plan_segments(dtype, N, n, (ctypes.c_long * n)(*Ms), (ctypes.c_long * n)(*Ks),
+                                        ctypes.byref(o) if o is not None else None, ctypes.byref(ns), ends, tiles))
+    return [(ends[s], tiles[s]) for s in range(ns.value)]
 
 
 def conv_dims(g: ConvGeom):
@@ -393,6 +406,14 @@ def grouped_fits(req: ChainReq) -> bool:
     if rc < 0:
         _check(rc)
     return rc == 1
+
+
+def grouped_tile(req: ChainReq) -> int:
+    """Tile width of the request in a grouped launch (32 or 64), 0 if it runs outside it."""
+    rc = lib().fg_grouped_tile(ctypes.byref(req))
+    if rc < 0:
+        _check(rc)
+    return rc
 
 
 class GroupedGraph:

@@ -185,6 +185,97 @@ int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bo
   return best;
 }
 
+// ---------------------------------------------------------------- segments --
+// A dependent kernel launch costs ~2.5 us back to back (~6000 cycles at 2.4 GHz).
+constexpr double LAUNCH_CYCLES = 6000.0;
+// HBM per CU and clock at 8 TB/s over 256 CUs at 2.4 GHz.
+constexpr double HBM_BYTES_PER_CU_CLK = 13.0;
+
+static double elem_bytes(int kind) { return kind == K_I8 ? 1 : (kind == K_BF16 ? 2 : 4); }
+
+// Modelled time (chip cycles) of one unfused GEMM: MFMA at ~60 % of the kind's peak
+// or its operand and output bytes at HBM speed, whichever is longer, plus a launch.
+static double gemm_cycles(int kind, long M, long N, long K, bool last) {
+  const double rate = kind == K_F32 ? 256.0 : (kind == K_F32X3 ? 4096.0 / 3 : (kind == K_BF16 ? 4096.0 : 8192.0));
+  const double e = elem_bytes(kind);
+  const double flops = 2.0 * M * N * K, bytes = e * ((double)M * K + (double)K * N) + (last ? 4.0 : e) * M * N;
+  const double mfma = flops / (num_cus() * rate * 0.6), hbm = bytes / (num_cus() * HBM_BYTES_PER_CU_CLK);
+  return (mfma > hbm ? mfma : hbm) + LAUNCH_CYCLES;
+}
+
+// Modelled time of a fused segment: fused_cost plus its input and output at HBM speed
+// (the fused kernel stages them before its first and after its last MFMA) and a launch.
+static double segment_cycles(int kind, long N, int n, const long *M, const long *K, int tn, bool last) {
+  const double e = elem_bytes(kind);
+  const double io = e * K[0] * N + (last ? 4.0 : e) * M[n - 1] * N;
+  return fused_cost(kind, N, n, M, K, tn) + io / (num_cus() * HBM_BYTES_PER_CU_CLK) + LAUNCH_CYCLES;
+}
+
+bool plan_valid(int kind, long N, int n, const long *M, const long *K, const ChainPlan &p) {
+  if (p.nseg < 1 || p.nseg > n || p.splits < 0) return false;
+  int start = 0;
+  for (int s = 0; s < p.nseg; ++s) {
+    const int end = p.end[s], len = end - start + 1;
+    if (end < start || end >= n) return false;
+    if (p.tile[s] == 0) {
+      if (len != 1) return false;
+    } else if (!chain_fused_tile_ok(kind, p.tile[s]) || !chain_fused_shape_ok(kind, N, len, K + start) ||
+               chain_fused_lds_bytes(kind, M + start, K + start, len, p.tile[s]) > LDS_BYTES_MAX) {
+      return false;
+    }
+    start = end + 1;
+  }
+  return start == n;
+}
+
+// Shortest path over GEMM boundaries: segment [i, j) is one unfused GEMM (j = i + 1)
+// or, for two GEMMs or more, a fused run at plan_tile_n when it fits LDS. The fewest
+// segments win (every boundary spills an intermediate to HBM and back, which is what
+// fusion saves), the modelled time breaks ties; the autotuner measures the rest.
+void plan_segments(int kind, long N, int n, const long *M, const long *K, ChainPlan *p) {
+  constexpr double PER_SEGMENT = 1e15;
+  double best[CHAIN_MAXG + 1];
+  int from[CHAIN_MAXG + 1], tile[CHAIN_MAXG + 1];
+  best[0] = 0;
+  for (int j = 1; j <= n; ++j) {
+    best[j] = 1e300;
+    for (int i = 0; i < j; ++i) {
+      const int len = j - i;
+      const bool last = j == n;
+      double c = 1e300;
+      int t = 0;
+      if (len == 1) c = gemm_cycles(kind, M[i], N, K[i], last);
+      const int tn = len >= 2 && chain_fused_shape_ok(kind, N, len, K + i) ? plan_tile_n(kind, N, len, M + i, K + i)
+                                                                         : 0;
+      if (tn > 0) {
+        const double cf = segment_cycles(kind, N, len, M + i, K + i, tn, last);
+        if (cf < c) {
+          c = cf;
+          t = tn;
+        }
+      }
+      c += PER_SEGMENT;
+      if (best[i] + c < best[j]) {
+        best[j] = best[i] + c;
+        from[j] = i;
+        tile[j] = t;
+      }
+    }
+  }
+  int ends[CHAIN_MAXG], tiles[CHAIN_MAXG], ns = 0;
+  for (int j = n; j > 0; j = from[j]) {
+    ends[ns] = j - 1;
+    tiles[ns] = tile[j];
+    ++ns;
+  }
+  p->nseg = ns;
+  for (int s = 0; s < ns; ++s) {  // back to chain order
+    p->end[s] = ends[ns - 1 - s];
+    p->tile[s] = tiles[ns - 1 - s];
+  }
+  p->splits = 0;
+}
+
 }  // namespace fg
 
 // ================================================================ part 2 ===
@@ -258,12 +349,15 @@ int fg_synchronize(void *stream) {
   return hip_status(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
 }
 
-static int validate_chain(int dtype, long N, int num_gemm, const void *const *A, const long *M, const long *K) {
+// check_a: the weight pointers are part of the call (not for plan inspection).
+static int validate_chain(int dtype, long N, int num_gemm, const void *const *A, const long *M, const long *K,
+                          bool check_a = true) {
   if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8) return set_error(FG_ERR_INVALID, "dtype %d", dtype);
   if (N <= 0) return set_error(FG_ERR_INVALID, "N = %ld", N);
   if (num_gemm < 1 || num_gemm > 8) return set_error(FG_ERR_INVALID, "num_gemm = %d (1..8)", num_gemm);
+  if (!M || !K) return set_error(FG_ERR_INVALID, "M or K is NULL");
   for (int i = 0; i < num_gemm; ++i) {
-    if (!A || !A[i]) return set_error(FG_ERR_INVALID, "A[%d] is NULL", i);
+    if (check_a && (!A || !A[i])) return set_error(FG_ERR_INVALID, "A[%d] is NULL", i);
     if (M[i] <= 0 || K[i] <= 0) return set_error(FG_ERR_INVALID, "gemm %d: M=%ld K=%ld", i + 1, M[i], K[i]);
     if (i > 0 && K[i] != M[i - 1])
       return set_error(FG_ERR_INVALID, "gemm %d: K=%ld but the previous M=%ld", i + 1, K[i], M[i - 1]);
@@ -299,9 +393,13 @@ static int ref_panels(long N, int threads, RefPanels *p) {
 // intermediates, *refl = 1) when every reference panel is a 32-aligned block and every
 // intermediate M_i a multiple of 32 (chain_refl_ok), without epilogue extensions or an
 // image B (img); otherwise it runs unfused with a gather after each GEMM.
+// The plan: one fused segment (the whole chain on chip), or, when it does not fit,
+// the segments of plan_segments (mathematical layout, plain epilogues, matrix B) or one
+// unfused GEMM per segment (the reference layout's gather path, epilogue extensions,
+// options that ask for it). An autotune entry for the chain's shape bucket decides
+// first when the options leave the choice open and it can run this exact shape.
 static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
-                      int *fused, int *tile_n, int *splits, bool epi = false, bool img = false,
-                      int *refl_out = nullptr) {
+                      ChainPlan *pl, bool epi = false, bool img = false, int *refl_out = nullptr) {
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
   int kind = kind_of(dtype, o.f32_mode);
@@ -313,17 +411,13 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   }
   if (refl_out) *refl_out = refl;
   const bool ref = any_ref && !refl;  // the reference layout through the unfused gather path
-  *splits = o.splits;
-  if (!any_ref && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
-    int f, t, sp;
-    // an entry that cannot run this exact shape (a tile width the kind lacks, or an
+  pl->splits = o.splits;
+  if (!any_ref && !epi && !img && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 && o.splits == 0) {
+    // an entry that cannot run this exact shape (a tile width the kind lacks, an
     // arena beyond LDS) is ignored: the heuristics below decide instead
-    if (autotune_lookup(kind, N, num_gemm, M, K, &f, &t, &sp) &&
-        (!f || (chain_fused_tile_ok(kind, t) && chain_fused_shape_ok(kind, N, num_gemm, K) &&
-                chain_fused_lds_bytes(kind, M, K, num_gemm, t, epi) <= LDS_BYTES_MAX))) {
-      *fused = f;
-      *tile_n = f ? t : 0;
-      *splits = f ? 0 : sp;
+    ChainPlan t;
+    if (autotune_lookup_plan(kind, N, num_gemm, M, K, &t) && plan_valid(kind, N, num_gemm, M, K, t)) {
+      *pl = t;
       return FG_OK;
     }
   }
@@ -340,15 +434,52 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   if (o.fuse == FG_FUSE_ON && tn == 0)
     return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but %s",
                      ref ? "the reference layout runs unfused" : "no tile width fits LDS");
-  *fused = tn > 0;
-  *tile_n = tn;
+  if (tn > 0) {
+    pl->nseg = 1;
+    pl->end[0] = num_gemm - 1;
+    pl->tile[0] = tn;
+    return FG_OK;
+  }
+  if (!ref && !epi && !img && o.fuse == FG_FUSE_AUTO && o.tile_n == 0 &&
+      chain_fused_shape_ok(kind, N, num_gemm, K)) {
+    const int splits = pl->splits;
+    plan_segments(kind, N, num_gemm, M, K, pl);
+    pl->splits = splits;
+    return FG_OK;
+  }
+  pl->nseg = num_gemm;
+  for (int i = 0; i < num_gemm; ++i) {
+    pl->end[i] = i;
+    pl->tile[i] = 0;
+  }
   return FG_OK;
 }
 
+static bool whole_fused(const ChainPlan &p) { return p.nseg == 1 && p.tile[0] > 0; }
+
 int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
                   int *fused, int *tile_n) {
-  int splits;
-  return plan_chain(dtype, N, num_gemm, M, K, opts, fused, tile_n, &splits);
+  ChainPlan p;
+  int rc = plan_chain(dtype, N, num_gemm, M, K, opts, &p);
+  if (rc) return rc;
+  *fused = whole_fused(p);
+  *tile_n = *fused ? p.tile[0] : 0;
+  return FG_OK;
+}
+
+int fg_chain_plan_segments(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
+                           int *num_segments, int *seg_end, int *seg_tile) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if ((rc = validate_chain(dtype, N, num_gemm, nullptr, M, K, false))) return rc;
+  ChainPlan p;
+  if ((rc = plan_chain(dtype, N, num_gemm, M, K, opts, &p))) return rc;
+  *num_segments = p.nseg;
+  for (int s = 0; s < p.nseg; ++s) {
+    seg_end[s] = p.end[s];
+    seg_tile[s] = p.tile[s];
+  }
+  return FG_OK;
 }
 
 // One GEMM (+ INT8 column sums, + split-K reduction) of the unfused path. With
@@ -450,6 +581,49 @@ static int run_fused(ChainArgs c, int a_packed, hipStream_t s) {
   return hip_status(launch_chain_fused(c, s), "fused chain");
 }
 
+}  // extern "C"
+
+namespace fg {
+// Segment s writes its output (the next operand, requantized as the unfused epilogue
+// does) to spill buffer s % 2 and reads its input from the other one (or B).
+int run_plan(int kind, int dtype, const ChainPlan &p, void *C, const void *B, long N, int n, const void *const *A,
+             const long *M, const long *K, int a_packed, hipStream_t s) {
+  int rc;
+  if (!plan_valid(kind, N, n, M, K, p)) return set_error(FG_ERR_INVALID, "chain plan does not fit this chain");
+  for (int q = 0; q < p.nseg; ++q)
+    if (a_packed && p.tile[q] == 0) return set_error(FG_ERR_UNSUPPORTED, "packed weights need the fused path");
+  const size_t e = dtype_bytes(dtype);
+  long maxm = 0;
+  for (int q = 0; q + 1 < p.nseg; ++q) maxm = M[p.end[q]] > maxm ? M[p.end[q]] : maxm;
+  void *buf[2] = {nullptr, nullptr};
+  if (p.nseg > 1)
+    for (int b = 0; b < 2; ++b)
+      if ((rc = scratch(s, b ? "spill1" : "spill0", (size_t)maxm * N * e, &buf[b]))) return rc;
+  const void *X = B;
+  int g0 = 0;
+  for (int q = 0; q < p.nseg; ++q) {
+    const int g1 = p.end[q];
+    const bool last = g1 == n - 1;
+    void *Y = last ? C : buf[q & 1];
+    if (p.tile[q] > 0) {
+      ChainArgs c = chain_args(kind, X, Y, N, g1 - g0 + 1, A + g0, M + g0, K + g0, p.tile[q]);
+      c.out_requant = !last;
+      ChainParams P;
+      make_chain_params(c, &P);
+      if (!chain_fused_aligned(P)) return set_error(FG_ERR_UNSUPPORTED, "fused segment needs B aligned to 4 elements");
+      if ((rc = run_fused(c, a_packed, s))) return rc;
+    } else if ((rc = run_gemm(kind, last ? EPI_STORE : EPI_REQUANT, A[g0], X, Y, M[g0], N, K[g0], p.splits, s))) {
+      return rc;
+    }
+    X = Y;
+    g0 = g1 + 1;
+  }
+  return FG_OK;
+}
+}  // namespace fg
+
+extern "C" {
+
 // Epilogue extensions from the public struct; *out = nullptr when every field is at
 // its default (the reference's epilogues, the plain kernels).
 static int epi_params(int dtype, int num_gemm, const fg_epilogue *e, EpiParams *p, const EpiParams **out) {
@@ -504,12 +678,13 @@ static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, c
   RefPanels panels{};
   if (ref && (rc = ref_panels(N, o.ref_threads, &panels))) return rc;
 
-  int fused = 0, tn = 0;
-  int splits = 0;
+  ChainPlan pl;
   int refl = 0;
-  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr, false, &refl))) return rc;
-  if (fused) {
-    ChainArgs c = chain_args(kind, B, C, N, num_gemm, A, M, K, tn);
+  if ((rc = plan_chain(dtype, N, num_gemm, M, K, &o, &pl, ep != nullptr, false, &refl))) return rc;
+  const int splits = pl.splits;
+  const bool b_aligned = N % 4 == 0 && ((uintptr_t)B % (4 * dtype_bytes(dtype))) == 0;
+  if (whole_fused(pl)) {
+    ChainArgs c = chain_args(kind, B, C, N, num_gemm, A, M, K, pl.tile[0]);
     c.epi = ep;
     c.refl = refl;
     ChainParams P;
@@ -518,6 +693,9 @@ static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, c
     if (o.fuse == FG_FUSE_ON || o.a_packed)
       return set_error(FG_ERR_UNSUPPORTED, "fused chain requested but B is not aligned to 4 elements");
     // misaligned B: run unfused
+  } else if (pl.nseg < num_gemm && (b_aligned || pl.tile[0] == 0)) {
+    // fused runs where they fit, spills between (mathematical layout, plain epilogues)
+    return run_plan(kind, dtype, pl, C, B, N, num_gemm, A, M, K, o.a_packed, s);
   }
   if (o.a_packed) return set_error(FG_ERR_UNSUPPORTED, "packed weights need the fused path");
 
@@ -681,11 +859,10 @@ int fg_chain_conv(int dtype, void *C, const void *X, const fg_conv_geom *g, int 
   const int kind = kind_of(dtype, o.f32_mode);
   // The gathering kernel loads single elements, so the fused path takes any N: plan
   // with N rounded up to whole 4-column groups.
-  int fused = 0, tn = 0, splits = 0;
-  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &fused, &tn, &splits, ep != nullptr, true)))
-    return rc;
-  if (fused) {
-    ChainArgs c = chain_args(kind, X, C, N, num_gemm, A, M, K, tn);
+  ChainPlan pl;
+  if ((rc = plan_chain(dtype, (N + 3) / 4 * 4, num_gemm, M, K, &o, &pl, ep != nullptr, true))) return rc;
+  if (whole_fused(pl)) {
+    ChainArgs c = chain_args(kind, X, C, N, num_gemm, A, M, K, pl.tile[0]);
     c.img = &ig;
     c.epi = ep;
     return run_fused(c, o.a_packed, s);

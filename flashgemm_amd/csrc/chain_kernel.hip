@@ -532,6 +532,31 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
     ls = (const float *)((const uint32_t *)((const char *)arena + P.epi_off) + rows) + off;
   }
   const bool has_bias = ex.bias != nullptr, has_scale = ex.scale != nullptr;
+  if (gi == P.num_gemm - 1 && (KIND == K_I8 || KIND == K_BF16) && P.out_requant) {
+    // a fused segment that ends inside a longer chain: its last GEMM writes the next
+    // GEMM's operand to HBM, requantized exactly as the unfused GEMM's epilogue does
+    // (int8 saturate, bf16 RNE), row-major M x N
+    const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
+#pragma unroll
+    for (int i = 0; i < RP; ++i) {
+      const int rbase = rb0 + i * 32;
+      if (rbase >= Mp) continue;
+#pragma unroll
+      for (int j = 0; j < TNW; ++j) {
+        const int n = (cb0 + j) * 32 + l32;
+        if (n >= tile_w) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int m = rbase + 4 * h + 8 * (r >> 2) + (r & 3);
+          if (m >= M) continue;
+          const size_t e = (size_t)m * P.N + n0 + n;
+          if (KIND == K_I8) ((int8_t *)P.C)[e] = sat_i8((int)acc[i][j][r]);
+          else ((uint16_t *)P.C)[e] = bf16_dev((float)acc[i][j][r]);
+        }
+      }
+    }
+    return;
+  }
   if (gi == P.num_gemm - 1) {
     const long tile_w = (P.N - n0 < TN) ? P.N - n0 : TN;
     const size_t rs = (size_t)P.N * 4;  // bytes per row of C
@@ -1022,6 +1047,8 @@ hipError_t launch_chain_t(const ChainArgs &c, hipStream_t s) {
   if (!c.img && !chain_fused_aligned(P)) return hipErrorInvalidValue;
   // reference-layout intermediates: INT8 / BF16 matrix chains with the plain epilogues
   if (c.refl && (c.img || c.epi || (KIND != K_I8 && KIND != K_BF16))) return hipErrorInvalidValue;
+  // a segment's requantized output takes the plain epilogues
+  if (c.out_requant && c.epi) return hipErrorInvalidValue;
   constexpr int TN = TNB * 32;
   const size_t lds = c.epi ? (size_t)P.epi_off + chain_epi_lds_bytes(c.M, c.num_gemm) : arena_bytes(KIND, P.S, TN);
   if (lds > LDS_LIMIT) return hipErrorInvalidValue;
@@ -1059,11 +1086,26 @@ __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restri
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const ChainParams &P = reqs[w.x];
+    // tiles of 32 columns, or of 64 where the autotune table chose them (gtnb = 2)
+    const long n0 = (long)w.y * P.gtnb * 32;
+    const bool wide = P.gtnb == 2;
     switch (P.kind) {
-      case K_F32: chain_tile<K_F32, GT_F32>(P, (long)w.y * GT_F32 * 32, arena); break;
-      case K_BF16: chain_tile<K_BF16, GT_BF16>(P, (long)w.y * GT_BF16 * 32, arena); break;
-      case K_I8: chain_tile<K_I8, GT_I8>(P, (long)w.y * GT_I8 * 32, arena); break;
-      case K_F32X3: chain_tile<K_F32X3, GT_F32X3>(P, (long)w.y * GT_F32X3 * 32, arena); break;
+      case K_F32:
+        if (wide) chain_tile<K_F32, 2>(P, n0, arena);
+        else chain_tile<K_F32, GT_F32>(P, n0, arena);
+        break;
+      case K_BF16:
+        if (wide) chain_tile<K_BF16, 2>(P, n0, arena);
+        else chain_tile<K_BF16, GT_BF16>(P, n0, arena);
+        break;
+      case K_I8:
+        if (wide) chain_tile<K_I8, 2>(P, n0, arena);
+        else chain_tile<K_I8, GT_I8>(P, n0, arena);
+        break;
+      case K_F32X3:
+        if (wide) chain_tile<K_F32X3, 2>(P, n0, arena);
+        else chain_tile<K_F32X3, GT_F32X3>(P, n0, arena);
+        break;
     }
     __syncthreads();  // the next tile restages the arena
   }
@@ -1194,6 +1236,7 @@ void make_chain_params(const ChainArgs &c, ChainParams *P) {
   if (c.img) P->img = *c.img;
   if (c.epi) P->epi = *c.epi;
   P->refl = c.refl;
+  P->out_requant = c.out_requant;
 }
 
 bool chain_refl_ok(int kind, const RefPanels &p, int num_gemm, const long *M) {

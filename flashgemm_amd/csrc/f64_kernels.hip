@@ -141,7 +141,31 @@ __global__ __launch_bounds__(512) void stream_kernel(const uint4 *__restrict__ r
   out[(size_t)blockIdx.x * 512 + threadIdx.x] = acc.x ^ acc.y ^ acc.z ^ acc.w;
 }
 
+// Seeded operands for timing runs (the autotuner): element i of a buffer from
+// splitmix64(seed + i): fp32 uniform in [-1, 1), bf16 the same rounded down to bf16,
+// int8 uniform in [-127, 127]. Timing on zeros would not load the MFMA pipes (nor
+// the clock) the way real data does.
+__global__ void fill_random_kernel(int esz, void *p, long n, unsigned long long seed) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)n; i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const float u = (float)(z >> 40) * (1.0f / 16777216.0f) * 2.0f - 1.0f;
+    if (esz == 4) ((float *)p)[i] = u;
+    else if (esz == 2) ((uint16_t *)p)[i] = (uint16_t)(__builtin_bit_cast(uint32_t, u) >> 16);
+    else ((int8_t *)p)[i] = (int8_t)((int)(z % 255u) - 127);
+  }
+}
+
 }  // namespace
+
+hipError_t launch_fill_random(int esz, void *p, long n, unsigned long long seed, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const long blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+  hipLaunchKernelGGL(fill_random_kernel, dim3((unsigned)blocks), dim3(256), 0, s, esz, p, n, seed);
+  return hipGetLastError();
+}
 
 // region: 8 * slice_frags KB, out: workgroups * 512 words.
 hipError_t launch_stream_probe(const void *region, unsigned slice_frags, int rounds, int workgroups, unsigned *out,

@@ -234,6 +234,8 @@ struct ChainArgs {
   const EpiParams *epi;     // nullptr: the reference's epilogues; else per-GEMM extensions
   int raw_a;                // A[i] are the row-major FP32 weights, read unpacked (chain_raw_weights_ok)
   int refl;                 // INT8 / BF16: intermediates in the reference's layout (chain_refl_ok)
+  int out_requant;          // the chain is a segment of a longer one: C receives the next
+                            // GEMM's operand (int8 / bf16 / fp32 M_n x N), not int32 / fp32 C
 };
 // The fused kernels can write INT8 / BF16 intermediates in the reference's layout
 // when every reference panel is a 32-aligned block (every thread's column range a
@@ -264,6 +266,8 @@ struct ChainParams {
   EpiParams epi;            // epilogue extensions (chain_kernel EPX)
   int epi_off;              // EPX: LDS byte offset of the per-row bias and scale tables
   int refl;                 // intermediates in the reference's layout (store_rows_refl)
+  int out_requant;          // INT8 / BF16: the last GEMM writes the next operand (int8 / bf16), not C
+  int gtnb;                 // grouped launch: 32-column blocks per tile of this request (1 or 2)
 };
 
 void make_chain_params(const ChainArgs &c, ChainParams *P);
@@ -320,6 +324,9 @@ hipError_t launch_f64_gemm(const double *A, const double *B, double *C, long M, 
 hipError_t launch_to_f64(int src_kind, const void *src, double *dst, long n, hipStream_t s);
 hipError_t launch_err_norms(int x_kind, const void *x, const double *ref, long n, double *out2,
                             hipStream_t s);
+// Seeded random operands of element size esz (4: fp32 in [-1, 1), 2: bf16, 1: int8 in
+// [-127, 127]) for timing runs.
+hipError_t launch_fill_random(int esz, void *p, long n, unsigned long long seed, hipStream_t s);
 // L2 -> CU stream-rate kernel (measurement): region of 8 * slice_frags KB, out of
 // workgroups * 512 words.
 hipError_t launch_stream_probe(const void *region, unsigned slice_frags, int rounds, int workgroups, unsigned *out,

@@ -4,7 +4,32 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include "fg_internal.h"
+
 namespace fg {
+
+// Execution plan of a chain: consecutive segments of GEMMs, each either one fused
+// launch at tile width tile[s] (its intermediates on chip; a segment that ends before
+// the chain does writes the next operand to HBM) or, with tile[s] == 0, one unfused
+// GEMM. One segment with a tile is the whole chain on chip.
+struct ChainPlan {
+  int nseg;
+  int end[CHAIN_MAXG];   // last GEMM (inclusive) of segment s
+  int tile[CHAIN_MAXG];  // fused tile width; 0: a single unfused GEMM
+  int splits;            // split-K of the unfused GEMMs (0: planner)
+};
+// True when `p` can run this exact shape: segments cover the chain in order, unfused
+// segments hold one GEMM, and every fused segment fits LDS at its tile width.
+bool plan_valid(int kind, long N, int n, const long *M, const long *K, const ChainPlan &p);
+// Model-based plan (no table): the whole chain fused when it fits; otherwise the
+// segmentation with the lowest modelled time (fused runs where they fit, spills between).
+void plan_segments(int kind, long N, int n, const long *M, const long *K, ChainPlan *p);
+// Autotune table (autotune.cpp): the measured plan of the chain's shape bucket.
+bool autotune_lookup_plan(int kind, long N, int n, const long *M, const long *K, ChainPlan *p);
+// Runs a plan on device pointers (mathematical layout, plain epilogues): fused
+// segments pack their weights unless a_packed (then every segment must be fused).
+int run_plan(int kind, int dtype, const ChainPlan &p, void *C, const void *B, long N, int n, const void *const *A,
+             const long *M, const long *K, int a_packed, hipStream_t s);
 
 int set_error(int status, const char *fmt, ...);
 int hip_status(hipError_t e, const char *what);
@@ -23,7 +48,5 @@ int sgemm_ws(float *C, const float *A, const float *B, long M, long N, long K, i
 int kind_of(int dtype, int f32_mode);
 int plan_splits(long M, long N, long K, int kind);
 int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bool epi = false);
-// Autotune table lookup (autotune.cpp): true when the shape has a measured decision.
-bool autotune_lookup(int kind, long N, int n, const long *M, const long *K, int *fused, int *tile_n, int *splits);
 
 }  // namespace fg

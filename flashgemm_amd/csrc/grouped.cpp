@@ -52,11 +52,23 @@ int describe(const fg_chain_req &r, ChainParams *P, int *tile_n, size_t *lds, bo
   c.B = r.B;
   c.C = r.C;
   c.N = r.N;
+  // Tile width: 32 columns, unless the autotune table has a plan for the request's
+  // shape bucket that runs this exact shape: the whole chain fused at 32 or 64
+  // columns sets the width; any other plan (wider tiles, segments that spill, unfused
+  // GEMMs) keeps the request out of the grouped launch, so it runs with that plan.
   *tile_n = grouped_tile_n(kind);
+  bool tuned_out = false;
+  ChainPlan tp;
+  if (autotune_lookup_plan(kind, r.N, r.num_gemm, r.M, r.K, &tp) && plan_valid(kind, r.N, r.num_gemm, r.M, r.K, tp)) {
+    if (tp.nseg == 1 && (tp.tile[0] == 32 || tp.tile[0] == 64)) *tile_n = tp.tile[0];
+    else tuned_out = true;
+  }
   c.tile_n = *tile_n;
   *lds = chain_fused_lds_bytes(kind, r.M, r.K, r.num_gemm, *tile_n);
   make_chain_params(c, P);
-  *fits = *lds <= LDS_BYTES_MAX && chain_fused_shape_ok(kind, r.N, r.num_gemm, r.K) && chain_fused_aligned(*P);
+  P->gtnb = *tile_n / 32;
+  *fits = !tuned_out && *lds <= LDS_BYTES_MAX && chain_fused_shape_ok(kind, r.N, r.num_gemm, r.K) &&
+          chain_fused_aligned(*P);
   return FG_OK;
 }
 
@@ -288,6 +300,16 @@ int fg_grouped_fits(const fg_chain_req *req) {
   int rc = describe(*req, &p, &tn, &l, &fits);
   if (rc) return rc;
   return fits ? 1 : 0;
+}
+
+int fg_grouped_tile(const fg_chain_req *req) {
+  ChainParams p;
+  int tn;
+  size_t l;
+  bool fits;
+  int rc = describe(*req, &p, &tn, &l, &fits);
+  if (rc) return rc;
+  return fits ? tn : 0;
 }
 
 int fg_grouped_graph_create(int max_reqs, int max_tiles, fg_grouped_graph **out) {

@@ -169,6 +169,14 @@ int fg_pack_weight(int dtype, int f32_mode, const void *A, long M, long K, void 
  * fused = 1 if the whole chain runs in one fused launch, tile_n its tile width. */
 int fg_chain_plan(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
                   int *fused, int *tile_n);
+/* The full plan (no reference counterpart: the reference runs every chain panel by
+ * panel on chip, multigemm/src/flashgemm.c:123-148). A chain is cut into segments of
+ * consecutive GEMMs: segment s ends with GEMM seg_end[s] (0-based, inclusive) and runs
+ * fused at tile width seg_tile[s] (its intermediates on chip; a segment that ends
+ * before the chain does writes the next GEMM's operand to HBM), or, when seg_tile[s]
+ * is 0, as one unfused GEMM. seg_end and seg_tile hold num_gemm entries. */
+int fg_chain_plan_segments(int dtype, long N, int num_gemm, const long *M, const long *K, const fg_chain_opts *opts,
+                           int *num_segments, int *seg_end, int *seg_tile);
 
 /* ---- convolution front end: B produced from an image (im2col). The reference's
  * chains are convolution layers written as GEMMs: its test shapes take K_1 =
@@ -224,12 +232,17 @@ typedef struct fg_chain_req {
 } fg_chain_req;
 
 /* All requests in one persistent launch: each request's columns are cut into tiles
- * (32 columns wide) that run the whole chain on chip. Requests whose
- * intermediates do not fit on chip at that width run unfused on the same stream
- * after it. Asynchronous. */
+ * (32 columns wide, or 64 where the autotune table's plan for the request's shape
+ * bucket says so) that run the whole chain on chip. Requests whose intermediates do
+ * not fit on chip at that width, or whose tuned plan is not one fused launch at 32 or
+ * 64 columns, run with their own plan (fg_chain) on the same stream after it.
+ * Asynchronous. */
 int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream);
 /* 1 if the request runs on chip in a grouped launch, 0 if not, < 0 on error. */
 int fg_grouped_fits(const fg_chain_req *req);
+/* The grouped launch's tile width for the request (32 or 64), 0 if it runs outside
+ * the grouped launch, < 0 on error. */
+int fg_grouped_tile(const fg_chain_req *req);
 
 /* A captured hipGraph of [descriptor-table upload -> grouped kernel], replayed per
  * batch. Four captured copies rotate so a batch can be prepared while earlier ones
@@ -278,15 +291,21 @@ long fg_server_num_requests(fg_server *srv);
 void fg_server_destroy(fg_server *srv);
 
 /* ---- autotuner (replaces the reference's static blocking cost model,
- * singlegemm/src/set_block.h:8-200 and set_TmTn.h:10-58). Times every runnable
- * plan for one chain shape (fused at each tile width that fits LDS; unfused with
- * and without split-K), keeps the fastest in a table keyed by (dtype / FP32 mode,
- * M_i, K_i, floor(log2 N)) that fg_chain's planner consults when opts leave the
- * choice open. Outputs may be NULL. ---- */
+ * singlegemm/src/set_block.h:8-200 and set_TmTn.h:10-58). Times the runnable plans
+ * of one chain shape on seeded random data: the whole chain fused at each tile width
+ * that fits LDS, every cut of the chain into fused segments (the fusion depth: which
+ * intermediates stay on chip and which spill to HBM), and every GEMM unfused with and
+ * without split-K. The fastest plan is kept in a table keyed by the shape bucket
+ * (dtype / FP32 mode, chain length, ceil(log2) of N and of every M_i and K_i); the
+ * planner of fg_chain and the grouped launch (tile width, or whether a request joins
+ * it at all) consult it when opts leave the choice open and the plan can run the
+ * exact shape (fg_chain_plan_segments shows the result). fused / tile_n report
+ * whether the best plan is the whole chain in one fused launch. Outputs may be NULL. ---- */
 int fg_autotune_chain(int dtype, int f32_mode, long N, int num_gemm, const long *M, const long *K, void *stream,
                       int *fused, int *tile_n, int *splits, double *ms);
-int fg_autotune_save(const char *path); /* "flashgemm-autotune v1 gfx950" text table */
-int fg_autotune_load(const char *path); /* merges entries; rejects other versions   */
+int fg_autotune_save(const char *path); /* "flashgemm-autotune v2 gfx950" text table */
+int fg_autotune_load(const char *path); /* merges entries; refuses bad entries and
+                                           other versions, leaving the table as it was */
 int fg_autotune_clear(void);
 int fg_autotune_size(void);
 

@@ -33,13 +33,24 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+from flashgemm_amd import workloads as W  # noqa: E402  (imports no HIP code)
 
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "int8": 5033.2, "fp64": 78.6}  # dense TFLOP/s (TOPS)
 HBM_GBS = 8000.0
-# HBM bytes per launch of the main chain kernel: rocprofv3 FETCH_SIZE 6214.0 KiB x 1.9997
-# + WRITE_SIZE 8192.0 KiB x 1.0000, the scales being what the same counters read on a
-# plain 256 MiB device copy (bench.py --copy-calibration). Algorithmic bytes: 17.30 MB.
-TRAFFIC_MAIN = 21112871
+# The main chain kernel's rocprofv3 summary: its "Traffic per launch (bytes)" line is the
+# HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE passes, each scaled by what the
+# same counter reads on a plain 256 MiB device copy (bench.py --copy-calibration).
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "chain_kernel_f32_2chain_N8192.md")
+
+
+def main_traffic():
+    """(HBM bytes per launch, source) of the main kernel from its committed profile, or (None, reason)."""
+    import re
+    try:
+        m = re.search(r"Traffic per launch \(bytes\): (\d+)", open(TRAFFIC_PROFILE).read())
+    except OSError:
+        return None, "no profile summary"
+    return (int(m.group(1)), os.path.relpath(TRAFFIC_PROFILE, ROOT)) if m else (None, "no traffic line")
 
 
 def log(msg: str) -> None:
@@ -77,17 +88,10 @@ class Timer:
         return e0.elapsed_time(e1) / steps  # ms per call
 
 
-def uniform(torch, shape, gen, dtype=None):
-    x = torch.rand(shape, generator=gen, device="cuda", dtype=torch.float32) * 2 - 1
-    return x if dtype is None else x.to(dtype)
-
-
 def bench_main(torch, fg, timer, stream, steps, warmup):
-    gen = torch.Generator(device="cuda").manual_seed(1)
-    N, K1, M1, M2 = 8192, 256, 256, 256
-    B = uniform(torch, (K1, N), gen)
-    A1 = uniform(torch, (M1, K1), gen)
-    A2 = uniform(torch, (M2, M1), gen)
+    B, (A1, A2) = W.chain2_f32(torch)
+    K1, N = B.shape
+    M1, M2 = A1.shape[0], A2.shape[0]
     C = torch.empty((M2, N), device="cuda", dtype=torch.float32)
     fused, tn = fg.chain_plan(fg.DTYPE_F32, N, [M1, M2], [K1, M1])
 
@@ -124,9 +128,7 @@ def bench_main(torch, fg, timer, stream, steps, warmup):
 
 
 def bench_sgemm(torch, fg, timer, stream, M, N, K, mode, steps=20, warmup=3):
-    gen = torch.Generator(device="cuda").manual_seed(M + N + K)
-    A = uniform(torch, (M, K), gen)
-    B = uniform(torch, (K, N), gen)
+    A, B = W.sgemm_operands(torch, M, N, K)
     C = torch.empty((M, N), device="cuda", dtype=torch.float32)
     ms = timer.run(lambda: fg.sgemm(C, A, B, M, N, K, f32_mode=mode, stream=stream), steps, warmup)
     ref = A.double() @ B.double()
@@ -141,8 +143,8 @@ def bench_sgemm_t(torch, fg, timer, stream, M=2048, N=2048, K=2048, steps=20, wa
     """FP32 GEMM with transposed operand storage (fg_sgemm_t: NT, TN, TT) beside NN;
     the reference's dispatcher computes NN only (singlegemm/FP32_ID23.c:39-43)."""
     gen = torch.Generator(device="cuda").manual_seed(5)
-    A = uniform(torch, (M, K), gen)
-    B = uniform(torch, (K, N), gen)
+    A = W.uniform(torch, (M, K), gen)
+    B = W.uniform(torch, (K, N), gen)
     At, Bt = A.t().contiguous(), B.t().contiguous()
     C = torch.empty((M, N), device="cuda", dtype=torch.float32)
     flop = 2.0 * M * N * K
@@ -157,17 +159,12 @@ def bench_sgemm_t(torch, fg, timer, stream, M=2048, N=2048, K=2048, steps=20, wa
 def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
     """X(16384x512).W1(512x512).W2(512x1024).W3(1024x512) in the reference
     orientation: B = X^T (512 x 16384), A_i = W_i^T."""
-    gen = torch.Generator(device="cuda").manual_seed(3)
-    N, K1, Ms = 16384, 512, [512, 1024, 512]
+    B, As = W.chain3(torch, dtype)
+    K1, N = B.shape
+    Ms = [a.shape[0] for a in As]
     Ks = [K1] + Ms[:-1]
-    if dtype == fg.DTYPE_INT8:
-        B = torch.randint(-127, 128, (K1, N), generator=gen, device="cuda", dtype=torch.int8)
-        As = [torch.randint(-127, 128, (m, k), generator=gen, device="cuda", dtype=torch.int8) for m, k in zip(Ms, Ks)]
-        C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.int32)
-    else:
-        B = (torch.randn((K1, N), generator=gen, device="cuda") / K1 ** 0.5).to(torch.bfloat16)
-        As = [(torch.randn((m, k), generator=gen, device="cuda") / k ** 0.5).to(torch.bfloat16) for m, k in zip(Ms, Ks)]
-        C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float32)
+    C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.int32 if dtype == fg.DTYPE_INT8 else torch.float32)
+    gen = torch.Generator(device="cuda").manual_seed(33)  # epilogue-extension operands
     fused, tn = fg.chain_plan(dtype, N, Ms, Ks)
     ms = timer.run(lambda: fg.chain(dtype, C, B, As, stream=stream), steps, warmup)
     ms_kernel = None
@@ -251,36 +248,18 @@ def bench_conv(torch, fg, timer, stream, dtype, channels, size, ksize, Ms, steps
                 fused=fg.chain_plan(dtype, (N + 3) // 4 * 4, Ms, Ks)[0])
 
 
-def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0), seed=5):
+def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0, 40000.0), seed=5):
     """10,000 chain requests with Poisson arrivals at a fixed rate against 64 resident
     weight sets: chain length uniform in 2..4, widths from {64 .. 2048}, dtype mix 50%
     BF16 / 30% INT8 / 20% FP32, request width (the reference's N, tokens) log-uniform
     in 1..4096, HBM admission budget 200 GB. The same trace runs through the engine
-    (continuous batching, grouped persistent kernel replayed from a hipGraph) and
-    through the per-GEMM baseline (each request alone, one launch per GEMM)."""
+    (continuous batching: requests that fit on chip in the grouped persistent kernel
+    replayed from a hipGraph, the others in one grouped launch per chain level) and
+    through the per-GEMM baseline (each request alone, one launch per GEMM), at 5k,
+    20k and 40k requests/s offered."""
     import numpy as np
-    rng = np.random.default_rng(seed)
-    dims = [64, 128, 256, 512, 1024, 2048]
-    dts = rng.permutation([fg.DTYPE_BF16] * 32 + [fg.DTYPE_INT8] * 19 + [fg.DTYPE_F32] * 13)
-    gen = torch.Generator(device="cuda").manual_seed(seed)
-    sets = []
-    for dt in dts:
-        w = [int(rng.choice(dims)) for _ in range(int(rng.integers(2, 5)) + 1)]
-        As = []
-        for m, k in zip(w[1:], w[:-1]):
-            if dt == fg.DTYPE_INT8:
-                As.append(torch.randint(-127, 128, (m, k), generator=gen, device="cuda", dtype=torch.int8))
-            else:
-                a = torch.randn((m, k), generator=gen, device="cuda") / k ** 0.5
-                As.append(a.to(torch.bfloat16) if dt == fg.DTYPE_BF16 else a)
-        esz = 1 if dt == fg.DTYPE_INT8 else (2 if dt == fg.DTYPE_BF16 else 4)
-        Bbuf = torch.empty(w[0] * 4096 * esz, device="cuda", dtype=torch.uint8)  # shared by the set's requests
-        Bbuf.random_(0, 256, generator=gen) if dt == fg.DTYPE_INT8 else \
-            Bbuf.view(torch.bfloat16 if dt == fg.DTYPE_BF16 else torch.float32).normal_(0, 1, generator=gen)
-        sets.append(dict(dtype=int(dt), w=w, As=As, B=Bbuf))
-    set_of = rng.integers(0, len(sets), n_req)
-    Ns = np.minimum(np.exp(rng.uniform(0.0, np.log(4096.0), n_req)).astype(np.int64) + 1, 4096)
-    gaps = rng.exponential(1.0, n_req)  # unit-rate Poisson gaps, scaled per offered rate
+    sets = W.serving_sets(torch, seed=seed)
+    set_of, Ns, gaps = W.serving_trace(n_req, len(sets), seed=seed + 1)
     c_bytes = [sets[i]["w"][-1] * int(n) * 4 for i, n in zip(set_of, Ns)]
     offs = np.concatenate([[0], np.cumsum([(b + 255) // 256 * 256 for b in c_bytes])])
     arena = torch.empty(int(offs[-1]), device="cuda", dtype=torch.uint8)  # one C region per request
@@ -288,11 +267,31 @@ def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0), seed=5):
     Bs = [sets[i]["B"].data_ptr() for i in set_of]
     flop = sum(2.0 * int(n) * sum(m * k for m, k in zip(sets[i]["w"][1:], sets[i]["w"][:-1]))
                for i, n in zip(set_of, Ns))
+
+    def c_view(i):
+        s = sets[set_of[i]]
+        dt = torch.int32 if s["dtype"] == fg.DTYPE_INT8 else torch.float32
+        return arena[int(offs[i]):int(offs[i]) + c_bytes[i]].view(dt).view(s["w"][-1], int(Ns[i]))
+
+    # Verification sample: requests spread over the trace, checked after each mode's run
+    # of the first rate against (a) the same request launched alone (fg_chain, the same
+    # bits are required) and (b) for FP32 / BF16, the FP64 chain of the same inputs.
+    sample = np.sort(np.random.default_rng(seed + 2).choice(n_req, min(64, n_req), replace=False))
+
+    def f64_ref(i):
+        s, n = sets[set_of[i]], int(Ns[i])
+        X = s["B"][:s["w"][0] * n].view(s["w"][0], n).double()
+        for a in s["As"]:
+            X = a.double() @ X
+        return X
+
+    refs = {int(i): f64_ref(i) for i in sample if sets[set_of[i]]["dtype"] != fg.DTYPE_INT8}
     torch.cuda.synchronize()
     out = dict(requests=n_req, weight_sets=len(sets), total_gflop=flop / 1e9)
-    for qps in rates:
+    for qi, qps in enumerate(rates):
         arrivals = (np.cumsum(gaps) / qps).tolist()
         r = {}
+        kept = {}
         for mode in ("engine", "per_gemm_baseline"):
             srv = fg.Server(per_gemm_baseline=(mode != "engine"))
             try:
@@ -306,24 +305,58 @@ def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0), seed=5):
             r[mode] = dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
                            mean_ms=float(lat.mean()), requests_per_s=n_req / wall, tflops=flop / wall / 1e12,
                            wall_s=wall)
+            if qi == 0:
+                kept[mode] = {int(i): c_view(i).clone() for i in sample}
         r["p99_speedup_vs_per_gemm"] = r["per_gemm_baseline"]["p99_ms"] / r["engine"]["p99_ms"]
+        if qi == 0:
+            r["verified"] = verify_serving(torch, fg, sets, set_of, Ns, sample, kept, refs)
         out[f"qps_{int(qps)}"] = r
     return out
 
 
+def verify_serving(torch, fg, sets, set_of, Ns, sample, kept, refs):
+    """Sampled requests of a serving run: the engine's result equals the same request
+    launched alone through fg_chain with the engine's plan for it (default options
+    when it fits the on-chip grouped tile, else unfused without split-K: the level
+    launch) bit for bit; INT8 results of the engine and of the per-GEMM baseline are
+    equal; FP32 / BF16 relative Frobenius error against FP64 per path and dtype (the
+    p99 comparison is at these accuracies)."""
+    alone_same, int8_same = 0, 0
+    n_int8 = 0
+    errs = {"engine": {"bf16": [], "f32": []}, "per_gemm_baseline": {"bf16": [], "f32": []}}
+    for i in sample:
+        i = int(i)
+        s, n = sets[set_of[i]], int(Ns[i])
+        w = s["w"]
+        single = torch.empty_like(kept["engine"][i])
+        fits = fg.grouped_fits(fg.chain_req(s["dtype"], single, s["B"], s["As"], N=n, Ms=w[1:], Ks=w[:-1]))
+        o = None if fits else fg.opts(fuse=fg.FUSE_OFF, splits=1)
+        fg.chain(s["dtype"], single, s["B"], s["As"], N=n, Ms=w[1:], Ks=w[:-1], o=o)
+        torch.cuda.synchronize()
+        alone_same += int(torch.equal(single, kept["engine"][i]))
+        if s["dtype"] == fg.DTYPE_INT8:
+            n_int8 += 1
+            int8_same += int(torch.equal(kept["engine"][i], kept["per_gemm_baseline"][i]))
+            continue
+        ref = refs[i]
+        name = "bf16" if s["dtype"] == fg.DTYPE_BF16 else "f32"
+        for mode in errs:
+            got = kept[mode][i].double()
+            errs[mode][name].append(float(torch.linalg.norm(got - ref) / torch.linalg.norm(ref)))
+    rel = {mode: {k: dict(max=max(v), median=float(sorted(v)[len(v) // 2]), n=len(v)) if v else None
+                  for k, v in d.items()} for mode, d in errs.items()}
+    return dict(sampled=len(sample), same_bits_alone_and_in_the_mix=alone_same == len(sample),
+                alone_same_count=alone_same, int8_engine_equals_per_gemm=int8_same == n_int8, int8_sampled=n_int8,
+                rel_err_vs_fp64=rel)
+
+
 def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
-    gen = torch.Generator(device="cuda").manual_seed(4)
-    N, widths = 16384, [1024, 4096, 1024, 4096, 1024]
-
-    def scaled(rows, cols):
-        s = 10.0 ** (torch.rand((1, cols), generator=gen, device="cuda") * 6 - 3)
-        return (torch.randn((rows, cols), generator=gen, device="cuda") * s).contiguous()
-
-    B = scaled(widths[0], N)
-    As = [scaled(widths[i + 1], widths[i]) for i in range(4)]
+    B, As = W.precision_chain(torch)
+    N, widths = B.shape[1], list(W.PRECISION_WIDTHS)
     Ms, Ks = widths[1:], widths[:-1]
     ref = torch.empty((widths[-1], N), device="cuda", dtype=torch.float64)
-    fg.f64_chain(ref, B.double(), [a.double() for a in As], N=N, Ms=Ms, Ks=Ks, stream=stream)
+    Bd, Ad = B.double(), [a.double() for a in As]  # alive until the queued chain has run
+    fg.f64_chain(ref, Bd, Ad, N=N, Ms=Ms, Ks=Ks, stream=stream)
     out = {}
     flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
     for name, mode in (("fp32_mfma", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3)):
@@ -353,11 +386,51 @@ def stream_calibration(fg, stream, clock_ghz=2.4):
     return out
 
 
+def physical_cores():
+    """(physical cores, logical CPUs) among the CPUs this process may run on, from the
+    kernel's topology files (lscpu's CORE / SOCKET columns)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            cores.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+        except OSError:
+            cores.add(("cpu", str(c)))
+    return len(cores), len(cpus)
+
+
 def cpu_baseline(inputs, budget_s=12.0):
-    """The main workload on this host's cores: the compiled reference when it can
-    run here, else the oracle's C restatement."""
+    """The main workload on this host's cores, in a child process whose OpenMP runtime
+    starts with one thread per physical core, OMP_PROC_BIND=spread, OMP_PLACES=cores
+    (this process's OpenMP runtime was initialised by torch already)."""
+    import subprocess
+    import tempfile
+
     import numpy as np
-    B, A1, A2 = [t.cpu().numpy() for t in inputs]
+    cores, logical = physical_cores()
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if share > 0:  # the host's CPU share for this process, when the environment names one
+        cores = min(cores, share)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "inputs.npz")
+        np.savez(path, *[t.cpu().numpy() for t in inputs])
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="spread", OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", path, str(budget_s)],
+                           env=env, capture_output=True, text=True, timeout=budget_s * 4 + 120)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed: {r.stderr.strip()[-400:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    out["sample"] += f"; {cores} physical cores of {logical} logical CPUs available, OMP_PROC_BIND=spread OMP_PLACES=cores"
+    return out
+
+
+def cpu_baseline_child(path, budget_s):
+    """Times the main workload on the CPU (run by cpu_baseline with the OpenMP environment
+    set): the compiled reference when it can run here, else the oracle's C restatement."""
+    import numpy as np
+    z = np.load(path)
+    B, A1, A2 = z["arr_0"], z["arr_1"], z["arr_2"]
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     flop = 2.0 * B.shape[1] * (A1.shape[0] * A1.shape[1] + A2.shape[0] * A2.shape[1])
     flags = cpu_flags()
@@ -406,7 +479,12 @@ def main() -> None:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 5 device copies of 256 MiB (reads 256 MiB, writes 256 MiB each) and exit: "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--cpu-baseline-child", nargs=2, metavar=("INPUTS", "SECONDS"), default=None,
+                    help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_child:
+        print(json.dumps(cpu_baseline_child(args.cpu_baseline_child[0], float(args.cpu_baseline_child[1]))))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "design-only, see DESIGN.md)"}))
@@ -476,6 +554,7 @@ def main() -> None:
         except Exception as e:
             cpu = {"error": f"{type(e).__name__}: {e}"}
 
+    traffic, traffic_src = main_traffic()
     line = {
         "metric": "TFLOP/s+%peak per GEMM chain; serving p50/p99 vs plain kernel at stated error",
         "value": value,
@@ -496,7 +575,8 @@ def main() -> None:
                    "rel_err_vs_fp64": main_r["rel_err_vs_fp64"],
                    "ms_prepacked_weights": main_r["ms_kernel"]},
         "roofline": {"bound": "mfma", "achieved": kern_tf, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": kern_tf / PEAK["fp32"], "traffic": TRAFFIC_MAIN,
+                     "frac": kern_tf / PEAK["fp32"], "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": f"chain_kernel<K_F32, TN={main_r['tile_n']}, row-major weights>",
                      "kernel_us": kern_ms * 1e3,
                      "algorithmic_bytes_per_launch": main_r["bytes"],

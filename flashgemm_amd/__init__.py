@@ -142,6 +142,7 @@ def lib() -> ctypes.CDLL:
         L.fg_grouped_chain.argtypes = [i, ctypes.POINTER(ChainReq), vp]
         L.fg_grouped_fits.argtypes = [ctypes.POINTER(ChainReq)]
         L.fg_grouped_tile.argtypes = [ctypes.POINTER(ChainReq)]
+        L.fg_grouped_levels.argtypes = [i, ctypes.POINTER(ChainReq), vp]
         L.fg_chain_plan_segments.argtypes = [i, l, i, lp, lp, ctypes.POINTER(ChainOpts), ctypes.POINTER(i),
                                              ctypes.POINTER(i), ctypes.POINTER(i)]
         L.fg_grouped_graph_create.argtypes = [i, i, ctypes.POINTER(vp)]
@@ -399,6 +400,12 @@ def chain_req(dtype: int, C, B, As, N: int | None = None, f32_mode: int = F32_MF
 def grouped_chain(reqs, stream=None) -> None:
     arr = (ChainReq * len(reqs))(*reqs)
     _check(lib().fg_grouped_chain(len(reqs), arr, _stream(stream)))
+
+
+def grouped_levels(reqs, stream=None) -> None:
+    """Every request unfused, GEMM i of all of them in one launch (fg_grouped_levels)."""
+    arr = (ChainReq * len(reqs))(*reqs)
+    _check(lib().fg_grouped_levels(len(reqs), arr, _stream(stream)))
 
 
 def grouped_fits(req: ChainReq) -> bool:

@@ -1076,7 +1076,10 @@ constexpr int GT_F32 = 1, GT_BF16 = 1, GT_I8 = 1, GT_F32X3 = 1;
 // Persistent grouped launch over a packed table (GroupTable header, the request
 // descriptors, the tile list): workgroup b runs tiles b, b + grid, ...; each tile is
 // a whole chain on TN columns of its request. Counts and offsets are read from the
-// table, so one captured graph replays batches of any size.
+// table, so one captured graph replays batches of any size. WIDE adds the 64-column
+// tiles (gtnb = 2) the autotune table can choose; it is a separate instantiation
+// because its registers would cost the 32-column path occupancy and spills.
+template <bool WIDE>
 __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restrict__ table) {
   extern __shared__ uint4 arena[];
   const GroupTable *hdr = (const GroupTable *)table;
@@ -1086,26 +1089,21 @@ __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restri
   for (int it = blockIdx.x; it < n; it += gridDim.x) {
     const int2 w = items[it];
     const ChainParams &P = reqs[w.x];
-    // tiles of 32 columns, or of 64 where the autotune table chose them (gtnb = 2)
     const long n0 = (long)w.y * P.gtnb * 32;
-    const bool wide = P.gtnb == 2;
-    switch (P.kind) {
-      case K_F32:
-        if (wide) chain_tile<K_F32, 2>(P, n0, arena);
-        else chain_tile<K_F32, GT_F32>(P, n0, arena);
-        break;
-      case K_BF16:
-        if (wide) chain_tile<K_BF16, 2>(P, n0, arena);
-        else chain_tile<K_BF16, GT_BF16>(P, n0, arena);
-        break;
-      case K_I8:
-        if (wide) chain_tile<K_I8, 2>(P, n0, arena);
-        else chain_tile<K_I8, GT_I8>(P, n0, arena);
-        break;
-      case K_F32X3:
-        if (wide) chain_tile<K_F32X3, 2>(P, n0, arena);
-        else chain_tile<K_F32X3, GT_F32X3>(P, n0, arena);
-        break;
+    if (WIDE && P.gtnb == 2) {
+      switch (P.kind) {
+        case K_F32: chain_tile<K_F32, 2>(P, n0, arena); break;
+        case K_BF16: chain_tile<K_BF16, 2>(P, n0, arena); break;
+        case K_I8: chain_tile<K_I8, 2>(P, n0, arena); break;
+        case K_F32X3: chain_tile<K_F32X3, 2>(P, n0, arena); break;
+      }
+    } else {
+      switch (P.kind) {
+        case K_F32: chain_tile<K_F32, GT_F32>(P, n0, arena); break;
+        case K_BF16: chain_tile<K_BF16, GT_BF16>(P, n0, arena); break;
+        case K_I8: chain_tile<K_I8, GT_I8>(P, n0, arena); break;
+        case K_F32X3: chain_tile<K_F32X3, GT_F32X3>(P, n0, arena); break;
+      }
     }
     __syncthreads();  // the next tile restages the arena
   }
@@ -1265,12 +1263,12 @@ int grouped_tile_n(int kind) {
   return 0;
 }
 
-hipError_t launch_grouped(const void *table, int grid, size_t lds, hipStream_t s) {
+hipError_t launch_grouped(const void *table, int grid, size_t lds, bool wide, hipStream_t s) {
   if (lds > LDS_LIMIT || grid <= 0) return hipErrorInvalidValue;
-  hipError_t e = hipFuncSetAttribute((const void *)grouped_chain_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     (int)lds);
+  auto kfn = wide ? grouped_chain_kernel<true> : grouped_chain_kernel<false>;
+  hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(grouped_chain_kernel, dim3(grid), dim3(NTH), lds, s, (const char *)table);
+  hipLaunchKernelGGL(kfn, dim3(grid), dim3(NTH), lds, s, (const char *)table);
   return hipGetLastError();
 }
 

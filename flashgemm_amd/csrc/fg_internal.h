@@ -316,7 +316,32 @@ struct GroupTable {
   int items_off;  // bytes from the table start
 };
 int grouped_tile_n(int kind);
-hipError_t launch_grouped(const void *table, int grid, size_t lds, hipStream_t s);
+hipError_t launch_grouped(const void *table, int grid, size_t lds, bool wide, hipStream_t s);
+
+// Grouped unfused GEMMs: one launch runs the 128 x 128 output tiles of many
+// independent GEMMs (one level of many chains). Table: LevelTable header, GemmDesc[]
+// at descs_off, int2 {desc, tile} items at items_off; tile t of a descriptor is
+// (t / tiles_n, t % tiles_n). No split-K: every tile runs the whole K, so a GEMM's
+// result has the bits of the same GEMM launched alone without split-K.
+struct GemmDesc {
+  const void *A;  // M x K row-major
+  const void *B;  // K x N row-major
+  void *C;        // M x N: EPI_STORE fp32 / int32, EPI_REQUANT the next operand
+  long M, N, K;
+  int kind, epi, tiles_n, vec;  // vec: bit 0 A rows 16-byte aligned, bit 1 B columns vectorizable
+};
+struct LevelTable {
+  int num_descs;
+  int num_items;
+  int descs_off;  // bytes from the table start
+  int items_off;  // bytes from the table start
+};
+constexpr int LEVEL_TILE = 128;
+// Fills d (vector flags, tile counts) for C = A . B with the given kind and epilogue.
+void gemm_desc(GemmDesc *d, int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K);
+// One launch per level; each descriptor has its own kind and epilogue (EPI_STORE or
+// EPI_REQUANT). grid = the table's item count.
+hipError_t launch_grouped_gemm(const void *table, int grid, hipStream_t s);
 
 // FP64 error oracle (never on the product path).
 hipError_t launch_f64_gemm(const double *A, const double *B, double *C, long M, long N, long K,

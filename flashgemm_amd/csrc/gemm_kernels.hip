@@ -347,24 +347,20 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &
   tn = l / rows;
 }
 
-// TRANS bit 0: A is stored transposed (K x M, op(A) = A^T); bit 1: B is stored
-// transposed (N x K). FP32 kinds only.
+// One BT x BT output tile at (m0, n0) over k in [kbeg, kend), by one workgroup of
+// TL::NT threads; smem holds 4 x TL::TU4 slots ([stage][A|B]). EPI_PARTIAL writes
+// slab `slab` of the split-K partials. TRANS bit 0: A is stored transposed (K x M,
+// op(A) = A^T); bit 1: B is stored transposed (N x K). FP32 kinds only.
 template <int KIND, int EPI, class TL, int TRANS>
-__global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
-                                                      void *__restrict__ Cv, const int *__restrict__ colsum,
-                                                      long M, long N, long K, long kper, int tiles_m, int tiles_n,
-                                                      int vecA, int vecB) {
+__device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const void *__restrict__ Bv,
+                                          void *__restrict__ Cv, const int *__restrict__ colsum, long M, long N,
+                                          long K, long kbeg, long kend, long m0, long n0, int slab, int vecA,
+                                          int vecB, uint4 *smem) {
   using T = KT<KIND>;
   using Acc = typename T::Acc;
   constexpr int BT = TL::BT, IM = TL::IM, JN = TL::JN, TILE_U4 = TL::TU4;
-  extern __shared__ uint4 smem[];  // [stage][A|B], 4 x TILE_U4 slots
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave / TL::WN, wn = wave % TL::WN;
-  int tm, tn;
-  tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
-  const long m0 = (long)tm * BT, n0 = (long)tn * BT;
-  const long kbeg = (long)blockIdx.y * kper;
-  const long kend = (kbeg + kper < K) ? kbeg + kper : K;
 
   Acc acc[IM][JN];
 #pragma unroll
@@ -448,7 +444,7 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
       *(float *)p = (float)a[r];
     }
   };
-  char *const Cb = (char *)Cv + (EPI == EPI_PARTIAL ? (size_t)blockIdx.y * M * N * CE : 0);
+  char *const Cb = (char *)Cv + (EPI == EPI_PARTIAL ? (size_t)slab * M * N * CE : 0);
   if (m0 + BT <= M && n0 + BT <= N) {
     // whole tile inside C: one lane pointer per block, rows at uniform strides
     const size_t rs = (size_t)N * CE;
@@ -481,6 +477,58 @@ __global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ A
         put(Cb + ((size_t)m * N + n) * CE, acc[i][j], r, bias);
       }
   }
+}
+
+template <int KIND, int EPI, class TL, int TRANS>
+__global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
+                                                      void *__restrict__ Cv, const int *__restrict__ colsum,
+                                                      long M, long N, long K, long kper, int tiles_m, int tiles_n,
+                                                      int vecA, int vecB) {
+  extern __shared__ uint4 smem[];
+  int tm, tn;
+  tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const long kbeg = (long)blockIdx.y * kper;
+  const long kend = (kbeg + kper < K) ? kbeg + kper : K;
+  gemm_tile<KIND, EPI, TL, TRANS>(Av, Bv, Cv, colsum, M, N, K, kbeg, kend, (long)tm * TL::BT, (long)tn * TL::BT,
+                                  (int)blockIdx.y, vecA, vecB, smem);
+}
+
+// Grouped launch over a LevelTable: workgroup b runs item b, one 128 x 128 tile of one
+// GEMM over its whole K (the 256-thread tile of gemm_kernel, so a GEMM's bits match
+// its own unsplit launch), with the GEMM's kind and epilogue. One workgroup per item:
+// a persistent loop around the tile costs registers (INT8 to one wave per SIMD).
+__global__ __launch_bounds__(TileS::NT) void grouped_gemm_kernel(const char *__restrict__ table) {
+  extern __shared__ uint4 smem[];
+  const LevelTable *hdr = (const LevelTable *)table;
+  if ((int)blockIdx.x >= hdr->num_items) return;
+  const GemmDesc *descs = (const GemmDesc *)(table + hdr->descs_off);
+  const int2 w = ((const int2 *)(table + hdr->items_off))[blockIdx.x];
+  const GemmDesc &d = descs[w.x];
+  const int tm = w.y / d.tiles_n, tn = w.y - tm * d.tiles_n;
+  const long m0 = (long)tm * TileS::BT, n0 = (long)tn * TileS::BT;
+  const int va = d.vec & 1, vb = (d.vec >> 1) & 1;
+#define FG_LEVEL_TILE(KIND, EPI) \
+  gemm_tile<KIND, EPI, TileS, 0>(d.A, d.B, d.C, nullptr, d.M, d.N, d.K, 0, d.K, m0, n0, 0, va, vb, smem)
+  const bool last = d.epi == EPI_STORE;
+  switch (d.kind) {
+    case K_F32:
+      if (last) FG_LEVEL_TILE(K_F32, EPI_STORE);
+      else FG_LEVEL_TILE(K_F32, EPI_REQUANT);
+      break;
+    case K_BF16:
+      if (last) FG_LEVEL_TILE(K_BF16, EPI_STORE);
+      else FG_LEVEL_TILE(K_BF16, EPI_REQUANT);
+      break;
+    case K_I8:
+      if (last) FG_LEVEL_TILE(K_I8, EPI_STORE);
+      else FG_LEVEL_TILE(K_I8, EPI_REQUANT);
+      break;
+    case K_F32X3:
+      if (last) FG_LEVEL_TILE(K_F32X3, EPI_STORE);
+      else FG_LEVEL_TILE(K_F32X3, EPI_REQUANT);
+      break;
+  }
+#undef FG_LEVEL_TILE
 }
 
 // INT8 column sums sum_k s8(B[k][n]) for the u8 bias term (exact in int32 for
@@ -672,6 +720,42 @@ hipError_t launch_gemm(const GemmArgs &g, hipStream_t s) {
     case K_F32X3: return launch_kind<K_F32X3>(g, s);
   }
   return hipErrorInvalidValue;
+}
+
+template <int KIND>
+void gemm_desc_kind(GemmDesc *d, const void *A, const void *B, long N, long K) {
+  using T = KT<KIND>;
+  const bool va = (K * T::ESZ) % 16 == 0 && ((uintptr_t)A & 15) == 0;
+  const bool vb = N % T::CW == 0 && ((uintptr_t)B & (T::CW * T::ESZ - 1)) == 0;
+  d->vec = (va ? 1 : 0) | (vb ? 2 : 0);
+}
+
+void gemm_desc(GemmDesc *d, int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K) {
+  d->A = A;
+  d->B = B;
+  d->C = C;
+  d->M = M;
+  d->N = N;
+  d->K = K;
+  d->kind = kind;
+  d->epi = epi;
+  d->tiles_n = (int)((N + TileS::BT - 1) / TileS::BT);
+  switch (kind) {
+    case K_F32: gemm_desc_kind<K_F32>(d, A, B, N, K); break;
+    case K_BF16: gemm_desc_kind<K_BF16>(d, A, B, N, K); break;
+    case K_I8: gemm_desc_kind<K_I8>(d, A, B, N, K); break;
+    default: gemm_desc_kind<K_F32X3>(d, A, B, N, K); break;
+  }
+}
+
+hipError_t launch_grouped_gemm(const void *table, int grid, hipStream_t s) {
+  if (grid <= 0) return hipErrorInvalidValue;
+  const size_t lds = 4 * (size_t)TileS::TU4 * sizeof(uint4);
+  hipError_t e = hipFuncSetAttribute((const void *)grouped_gemm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(grouped_gemm_kernel, dim3(grid), dim3(TileS::NT), lds, s, (const char *)table);
+  return hipGetLastError();
 }
 
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s) {

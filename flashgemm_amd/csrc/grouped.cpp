@@ -27,6 +27,7 @@ struct Packed {
   std::vector<char> bytes;  // GroupTable + ChainParams[] + int2[]
   size_t lds = 0;
   int items = 0;
+  bool wide = false;  // some request runs 64-column tiles
 };
 
 size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
@@ -174,6 +175,7 @@ int pack(int count, const fg_chain_req *reqs, hipStream_t s, Packed *out, std::v
   for (size_t i = 0; i < items.size(); ++i) dst[i] = make_int2(items[i].req, items[i].tile);
   out->lds = lds;
   out->items = (int)items.size();
+  out->wide = std::any_of(P.begin(), P.end(), [](const ChainParams &p) { return p.gtnb == 2; });
   return FG_OK;
 }
 
@@ -278,7 +280,7 @@ int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream) {
     memcpy(sl.host, pk.bytes.data(), pk.bytes.size());
     if ((rc = hip_status(hipMemcpyAsync(sl.dev, sl.host, pk.bytes.size(), hipMemcpyHostToDevice, s), "H2D table")))
       return rc;
-    if ((rc = hip_status(launch_grouped(sl.dev, grouped_grid(pk.items), pk.lds, s), "grouped launch"))) return rc;
+    if ((rc = hip_status(launch_grouped(sl.dev, grouped_grid(pk.items), pk.lds, pk.wide, s), "grouped launch"))) return rc;
     if ((rc = hip_status(hipEventRecord(sl.done, s), "hipEventRecord"))) return rc;
   }
   for (int r : rest) {
@@ -290,6 +292,95 @@ int fg_grouped_chain(int count, const fg_chain_req *reqs, void *stream) {
     if ((rc = fg_chain(q.dtype, q.C, q.B, q.N, q.num_gemm, q.A, q.M, q.K, &o, stream))) return rc;
   }
   return FG_OK;
+}
+
+int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream) {
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  if (count < 0 || (count > 0 && !reqs)) return set_error(FG_ERR_INVALID, "count=%d", count);
+  if (count == 0) return FG_OK;
+  hipStream_t s = (hipStream_t)stream;
+  // validate; two ping-pong intermediates per request, in the next operand's type
+  std::vector<size_t> off(count), slot(count);
+  size_t total = 0;
+  int levels = 0;
+  for (int r = 0; r < count; ++r) {
+    const fg_chain_req &q = reqs[r];
+    if (q.dtype < FG_DTYPE_F32 || q.dtype > FG_DTYPE_INT8) return set_error(FG_ERR_INVALID, "request %d: dtype", r);
+    if (q.num_gemm < 1 || q.num_gemm > CHAIN_MAXG || q.N <= 0 || !q.B || !q.C)
+      return set_error(FG_ERR_INVALID, "request %d: num_gemm=%d N=%ld", r, q.num_gemm, q.N);
+    if (q.a_packed) return set_error(FG_ERR_UNSUPPORTED, "request %d: packed weights need the fused path", r);
+    long maxm = 0;
+    for (int i = 0; i < q.num_gemm; ++i) {
+      if (!q.A[i] || q.M[i] <= 0 || q.K[i] <= 0) return set_error(FG_ERR_INVALID, "request %d gemm %d", r, i + 1);
+      if (i > 0 && q.K[i] != q.M[i - 1])
+        return set_error(FG_ERR_INVALID, "request %d gemm %d: K != previous M", r, i + 1);
+      if (i + 1 < q.num_gemm) maxm = std::max(maxm, q.M[i]);
+    }
+    const size_t e = q.dtype == FG_DTYPE_INT8 ? 1 : q.dtype == FG_DTYPE_BF16 ? 2 : 4;
+    off[r] = total;
+    slot[r] = (maxm * q.N * e + 255) & ~(size_t)255;
+    total += 2 * slot[r];
+    levels = std::max(levels, q.num_gemm);
+  }
+  char *spill = nullptr;
+  if (total > 0 && (rc = scratch(s, "levels", total, (void **)&spill))) return rc;
+  // one table per level: descriptors with the longest tiles first (K over the kind's
+  // MFMA rate), then their tiles
+  auto tile_cost = [](const GemmDesc &d) {
+    const double rate = d.kind == K_I8 ? 32 : d.kind == K_BF16 ? 16 : d.kind == K_F32X3 ? 16.0 / 3 : 1;
+    return (double)d.K / rate;
+  };
+  struct Part {
+    int items;
+    size_t off;
+  };
+  std::vector<Part> parts;
+  std::vector<char> bytes;
+  for (int L = 0; L < levels; ++L) {
+    std::vector<GemmDesc> d;
+    for (int r = 0; r < count; ++r) {
+      const fg_chain_req &q = reqs[r];
+      if (q.num_gemm <= L) continue;
+      const bool last = L == q.num_gemm - 1;
+      char *x0 = spill ? spill + off[r] : nullptr, *x1 = x0 ? x0 + slot[r] : nullptr;
+      const void *Bin = L == 0 ? q.B : ((L - 1) & 1 ? x1 : x0);
+      void *Cout = last ? q.C : (L & 1 ? x1 : x0);
+      GemmDesc g;
+      gemm_desc(&g, kind_of(q.dtype, q.f32_mode), last ? EPI_STORE : EPI_REQUANT, q.A[L], Bin, Cout, q.M[L], q.N,
+                q.K[L]);
+      d.push_back(g);
+    }
+    std::stable_sort(d.begin(), d.end(),
+                     [&](const GemmDesc &a, const GemmDesc &b) { return tile_cost(a) > tile_cost(b); });
+    std::vector<int2> items;
+    for (size_t j = 0; j < d.size(); ++j) {
+      const long tiles = (d[j].M + LEVEL_TILE - 1) / LEVEL_TILE * d[j].tiles_n;
+      for (long t = 0; t < tiles; ++t) items.push_back(make_int2((int)j, (int)t));
+    }
+    LevelTable h{};
+    h.num_descs = (int)d.size();
+    h.num_items = (int)items.size();
+    h.descs_off = (int)align16(sizeof(LevelTable));
+    h.items_off = (int)align16(h.descs_off + d.size() * sizeof(GemmDesc));
+    const size_t at = (bytes.size() + 255) & ~(size_t)255;
+    bytes.resize(at + h.items_off + items.size() * sizeof(int2), 0);
+    memcpy(bytes.data() + at, &h, sizeof h);
+    memcpy(bytes.data() + at + h.descs_off, d.data(), d.size() * sizeof(GemmDesc));
+    memcpy(bytes.data() + at + h.items_off, items.data(), items.size() * sizeof(int2));
+    parts.push_back({h.num_items, at});
+  }
+  StreamRing *R = ring_for(s);
+  Slot &sl = R->slot[R->next];
+  R->next = (R->next + 1) % RING;
+  if ((rc = slot_reserve(sl, bytes.size()))) return rc;
+  memcpy(sl.host, bytes.data(), bytes.size());
+  if ((rc = hip_status(hipMemcpyAsync(sl.dev, sl.host, bytes.size(), hipMemcpyHostToDevice, s), "H2D level tables")))
+    return rc;
+  for (const Part &p : parts)
+    if ((rc = hip_status(launch_grouped_gemm((const char *)sl.dev + p.off, p.items, s), "grouped level launch")))
+      return rc;
+  return hip_status(hipEventRecord(sl.done, s), "hipEventRecord");
 }
 
 int fg_grouped_fits(const fg_chain_req *req) {
@@ -331,7 +422,7 @@ int fg_grouped_graph_create(int max_reqs, int max_tiles, fg_grouped_graph **out)
     // capture: table upload, then the grouped kernel (full LDS, one wave of CUs)
     if ((rc = hip_status(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture"))) break;
     hipError_t e1 = hipMemcpyAsync(g.slot.dev, g.slot.host, sizeof(GroupTable), hipMemcpyHostToDevice, cs);
-    hipError_t e2 = launch_grouped(g.slot.dev, num_cus(), LDS_BYTES_MAX, cs);
+    hipError_t e2 = launch_grouped(g.slot.dev, num_cus(), LDS_BYTES_MAX, false, cs);
     hipError_t e3 = hipStreamEndCapture(cs, &g.graph);
     if ((rc = hip_status(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3), "capture"))) break;
     size_t n = 0;
@@ -370,6 +461,17 @@ int fg_grouped_graph_launch(fg_grouped_graph *G, int count, const fg_chain_req *
   G->next = (G->next + 1) % RING;
   if ((rc = slot_reserve(g.slot, G->table_cap))) return rc;  // waits for this slot's last replay
   memcpy(g.slot.host, pk.bytes.data(), pk.bytes.size());
+  if (pk.wide) {
+    // the captured kernel runs 32-column tiles only: a batch with 64-column requests
+    // is launched directly from the same slot
+    hipStream_t s = (hipStream_t)stream;
+    if ((rc = hip_status(hipMemcpyAsync(g.slot.dev, g.slot.host, pk.bytes.size(), hipMemcpyHostToDevice, s),
+                         "H2D table")))
+      return rc;
+    if ((rc = hip_status(launch_grouped(g.slot.dev, num_cus(), LDS_BYTES_MAX, true, s), "grouped launch")))
+      return rc;
+    return hip_status(hipEventRecord(g.slot.done, s), "hipEventRecord");
+  }
   if ((rc = hip_status(hipGraphExecMemcpyNodeSetParams1D(g.exec, g.copy, g.slot.dev, g.slot.host,
                                                          pk.bytes.size(), hipMemcpyHostToDevice),
                        "update copy node")))

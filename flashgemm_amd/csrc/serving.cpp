@@ -9,7 +9,8 @@
 //     (replayed from a captured graph, or launched directly) on one of several
 //     streams, so requests that arrive while others run are admitted and their work
 //     overlaps on the device; requests too large for the grouped tile run unfused
-//     on the same stream;
+//     on the same stream, level by level (fg_grouped_levels: GEMM i of all of them
+//     in one launch);
 //   * completion is observed through events; latency = completion - arrival.
 // In per-GEMM mode the same scheduler runs each request alone with one launch per
 // GEMM (the plain-kernel baseline).
@@ -53,10 +54,14 @@ struct Request {
   int status;
 };
 
+// The requests of one part of a launched batch, completed when `ev` has: the on-chip
+// grouped launch and the level launch of one batch are two parts on one stream, so
+// the first part's requests retire without waiting for the second's work.
 struct Batch {
   std::vector<long> ids;
   hipEvent_t ev;
   int stream;
+  bool last;  // the batch's last part: its stream is free for another batch after it
 };
 
 double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
@@ -139,19 +144,21 @@ bool step(fg_server *S) {
   // grow the request and weight-set vectors while this thread launches.
   std::vector<fg_chain_req> q;
   int sidx = -1;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr, ev2 = nullptr;
   {
     std::lock_guard<std::mutex> lk(S->mu);
-    if (S->admitted.empty() || (int)S->inflight.size() >= S->cfg.num_streams) return did;
+    const int busy = (int)std::count_if(S->inflight.begin(), S->inflight.end(), [](const Batch &b) { return b.last; });
+    if (S->admitted.empty() || busy >= S->cfg.num_streams) return did;
     int take = std::min<int>((int)S->admitted.size(), S->cfg.max_batch);
     batch.assign(S->admitted.begin(), S->admitted.begin() + take);
     S->admitted.erase(S->admitted.begin(), S->admitted.begin() + take);
     sidx = S->next_stream;
     S->next_stream = (S->next_stream + 1) % S->cfg.num_streams;
-    if (!S->free_events.empty()) {
-      ev = S->free_events.back();
-      S->free_events.pop_back();
-    }
+    for (hipEvent_t *e : {&ev, &ev2})
+      if (!S->free_events.empty()) {
+        *e = S->free_events.back();
+        S->free_events.pop_back();
+      }
     q.resize(batch.size());
     for (size_t i = 0; i < batch.size(); ++i) {
       const Request &r = S->reqs[batch[i]];
@@ -175,9 +182,11 @@ bool step(fg_server *S) {
       }
     }
   }
-  if (!ev) (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  for (hipEvent_t *e : {&ev, &ev2})
+    if (!*e) (void)hipEventCreateWithFlags(e, hipEventDisableTiming);
   hipStream_t st = S->streams[sidx];
   int rc = FG_OK;
+  std::vector<long> head_ids;  // the first part (on-chip grouped launch) when both run
   if (S->cfg.per_gemm_baseline) {
     for (const fg_chain_req &x : q) {
       fg_chain_opts o;
@@ -190,9 +199,12 @@ bool step(fg_server *S) {
     }
   } else {
     std::vector<fg_chain_req> fit, rest;
+    std::vector<long> rest_ids;
     long tiles = 0;
-    for (const fg_chain_req &x : q) {
+    for (size_t i = 0; i < q.size(); ++i) {
+      const fg_chain_req &x = q[i];
       (x.a_packed ? fit : rest).push_back(x);
+      (x.a_packed ? head_ids : rest_ids).push_back(batch[i]);
       if (x.a_packed) tiles += (x.N + grouped_tile_n(kind_of(x.dtype, x.f32_mode)) - 1) /
                                grouped_tile_n(kind_of(x.dtype, x.f32_mode));
     }
@@ -203,8 +215,14 @@ bool step(fg_server *S) {
         rc = fg_grouped_graph_launch(S->graphs[sidx], (int)fit.size(), fit.data(), st);
       else
         rc = fg_grouped_chain((int)fit.size(), fit.data(), st);
+      if (!rc && !rest.empty()) rc = hip_status(hipEventRecord(ev2, st), "hipEventRecord");
     }
-    if (!rc && !rest.empty()) rc = fg_grouped_chain((int)rest.size(), rest.data(), st);
+    // the others run unfused, level by level: GEMM i of all of them in one launch, so
+    // their tiles fill the chip together instead of a few CUs per request and one
+    // launch per GEMM
+    if (!rc && !rest.empty()) rc = fg_grouped_levels((int)rest.size(), rest.data(), st);
+    if (fit.empty() || rest.empty()) head_ids.clear();  // one part
+    else batch = rest_ids;
   }
   if (!rc) rc = hip_status(hipEventRecord(ev, st), "hipEventRecord");
   std::lock_guard<std::mutex> lk(S->mu);
@@ -214,8 +232,11 @@ bool step(fg_server *S) {
       S->error_text = fg_last_error();
     }
     for (long id : batch) S->reqs[id].status = rc;
+    for (long id : head_ids) S->reqs[id].status = rc;
   }
-  S->inflight.push_back({batch, ev, sidx});
+  if (!head_ids.empty()) S->inflight.push_back({head_ids, ev2, sidx, false});
+  else S->free_events.push_back(ev2);
+  S->inflight.push_back({batch, ev, sidx, true});
   return true;
 }
 
@@ -330,7 +351,10 @@ int fg_server_add_weights(fg_server *S, int dtype, int f32_mode, int num_gemm, c
   return FG_OK;
 }
 
-int fg_server_submit(fg_server *S, int set_id, long N, const void *B, void *C, long *req_id) {
+namespace {
+// Latency is measured from `arrival`: the submit call's entry, or a trace's due time
+// (so time a late submitter spends behind the lock counts).
+int submit_at(fg_server *S, int set_id, long N, const void *B, void *C, long *req_id, Clock::time_point arrival) {
   if (!S || !B || !C || N <= 0) return set_error(FG_ERR_INVALID, "submit");
   std::lock_guard<std::mutex> lk(S->mu);
   if (set_id < 0 || set_id >= (int)S->sets.size()) return set_error(FG_ERR_INVALID, "weight set %d", set_id);
@@ -343,8 +367,8 @@ int fg_server_submit(fg_server *S, int set_id, long N, const void *B, void *C, l
   r.B = B;
   r.C = C;
   r.bytes = (size_t)w.K[0] * N * e + (size_t)w.M[w.num_gemm - 1] * N * 4;
-  // A request that cannot ride a grouped launch runs alone and takes library scratch
-  // (spill buffers, split-K slabs, packed weights): that is charged too.
+  // A request that cannot ride a grouped launch runs alone (without split-K) and takes
+  // library scratch (spill buffers, packed weights): that is charged too.
   fg_chain_req x;
   memset(&x, 0, sizeof x);
   x.dtype = w.dtype;
@@ -361,16 +385,21 @@ int fg_server_submit(fg_server *S, int set_id, long N, const void *B, void *C, l
   const int fits = S->cfg.per_gemm_baseline ? 0 : fg_grouped_fits(&x);
   if (fits < 0) return fits;
   if (!fits)
-    r.bytes += chain_scratch_bound(w.dtype, w.f32_mode, N, w.num_gemm, w.M, w.K, S->cfg.per_gemm_baseline != 0);
+    r.bytes += chain_scratch_bound(w.dtype, w.f32_mode, N, w.num_gemm, w.M, w.K, true);
   if (r.bytes + S->weight_bytes > S->cfg.hbm_budget_bytes)
     return set_error(FG_ERR_BUDGET, "request of %zu bytes can never fit the budget", r.bytes);
-  r.arrival = Clock::now();
+  r.arrival = arrival;
   r.status = FG_OK;
   S->reqs.push_back(r);
   S->pending.push_back(r.id);
   S->work_cv.notify_one();
   if (req_id) *req_id = r.id;
   return FG_OK;
+}
+}  // namespace
+
+int fg_server_submit(fg_server *S, int set_id, long N, const void *B, void *C, long *req_id) {
+  return submit_at(S, set_id, N, B, C, req_id, Clock::now());
 }
 
 int fg_server_drain(fg_server *S, double timeout_s) {
@@ -393,7 +422,7 @@ int fg_server_run_trace(fg_server *S, int n, const int *set_ids, const long *Ns,
       if (due - Clock::now() > std::chrono::microseconds(200))
         std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
-    int rc = fg_server_submit(S, set_ids[i], Ns[i], Bs[i], Cs[i], nullptr);
+    int rc = submit_at(S, set_ids[i], Ns[i], Bs[i], Cs[i], nullptr, due);
     if (rc) return rc;
   }
   return fg_server_drain(S, timeout_s);

@@ -243,6 +243,15 @@ int fg_grouped_fits(const fg_chain_req *req);
 /* The grouped launch's tile width for the request (32 or 64), 0 if it runs outside
  * the grouped launch, < 0 on error. */
 int fg_grouped_tile(const fg_chain_req *req);
+/* Every request unfused, level by level: GEMM i of all requests (i = 1 .. the longest
+ * chain, any mix of dtypes and modes) runs in one launch, 128 x 128 output tiles of
+ * all of them in one grid, each tile over its whole K; intermediates go to the
+ * stream's scratch in the next operand's type (as fg_chain's unfused path writes
+ * them). Each request's result has the bits of fg_chain with fuse = FG_FUSE_OFF and
+ * splits = 1. For batches of requests too large for the on-chip grouped launch: one
+ * launch per level instead of one per GEMM per request. Weights as given (a_packed
+ * must be 0). Asynchronous. */
+int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream);
 
 /* A captured hipGraph of [descriptor-table upload -> grouped kernel], replayed per
  * batch. Four captured copies rotate so a batch can be prepared while earlier ones

@@ -491,7 +491,7 @@ int fg_chain_plan_segments(int dtype, long N, int num_gemm, const long *M, const
 // that needs more than ws_bytes is refused with FG_ERR_INVALID.
 static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K, int splits,
                     hipStream_t s, const EpiOne *ex = nullptr, int trans = 0, void *ws = nullptr,
-                    size_t ws_bytes = 0) {
+                    size_t ws_bytes = 0, int kgroups = 0) {
   int *colsum = nullptr;
   int rc;
   auto partials = [&](size_t bytes, void **p) -> int {
@@ -516,7 +516,7 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
   }
   if (splits <= 0) splits = plan_splits(M, N, K, kind);
   if (splits <= 1 && !ex) {  // INT8: the kernel sums its own B columns (colsum == nullptr)
-    GemmArgs g{kind, epi, A, B, C, nullptr, M, N, K, 1, trans};
+    GemmArgs g{kind, epi, A, B, C, nullptr, M, N, K, 1, trans, kgroups};
     return hip_status(launch_gemm(g, s), "gemm");
   }
   if (kind == K_I8) {  // split-K slabs see part of K: the reduction adds the full column sums
@@ -740,7 +740,7 @@ int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, i
   if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
   int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
-  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream);
+  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, 0, nullptr, 0, 1);
 }
 
 }  // extern "C"
@@ -755,7 +755,7 @@ int sgemm_ws(float *C, const float *A, const float *B, long M, long N, long K, i
   if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
   const int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
-  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, 0, ws, ws_bytes);
+  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, 0, ws, ws_bytes, 1);
 }
 }  // namespace fg
 
@@ -770,7 +770,8 @@ int fg_sgemm_t(int transa, int transb, float *C, const float *A, const float *B,
   if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1))
     return set_error(FG_ERR_INVALID, "sgemm: transa=%d transb=%d (0 or 1)", transa, transb);
   int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
-  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, transa | (transb << 1));
+  return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, transa | (transb << 1),
+                  nullptr, 0, 1);
 }
 
 // ------------------------------------------------------- convolution front end --

@@ -189,6 +189,7 @@ struct GemmArgs {
   long M, N, K;
   int splits;       // split-K factor (EPI_PARTIAL writes splits slabs of M x N)
   int trans;        // FP32 kinds, EPI_STORE / EPI_PARTIAL: bit 0 A stored K x M, bit 1 B stored N x K
+  int kgroups;      // 1: a single FP32 GEMM may split each tile's k over two wave groups
 };
 
 hipError_t launch_gemm(const GemmArgs &g, hipStream_t s);

@@ -6,7 +6,10 @@
 //   * two tile shapes: 256 threads = 4 waves in a 2 x 2 grid on a 128 x 128 output
 //     tile (each wave 64 x 64 = 2 x 2 MFMA 32x32 blocks), and for large problems
 //     512 threads = 8 waves in a 2 x 4 grid on a 256 x 256 tile (each wave
-//     128 x 64 = 4 x 2 blocks), which halves the operand bytes per MFMA;
+//     128 x 64 = 4 x 2 blocks), which halves the operand bytes per MFMA; a single
+//     FP32 GEMM with at most one 128 x 128 tile per CU (2048^3) runs each tile with
+//     two groups of 4 waves splitting its k stages (512 threads, two waves per SIMD),
+//     the halves added in a fixed order through LDS;
 //   * A and B staged global -> registers -> LDS, double buffered, in the MFMA
 //     operand layout: 16-byte k-groups per row (A) or per column (B), slot index
 //     g*rows + (r ^ (g & 7)) so both the staging writes and the fragment reads
@@ -20,6 +23,7 @@
 #include <type_traits>
 
 #include "fg_internal.h"
+#include "fg_runtime.h"
 
 namespace fg {
 namespace {
@@ -348,19 +352,27 @@ __device__ __forceinline__ void tile_of(int bid, int tiles_m, int tiles_n, int &
 }
 
 // One BT x BT output tile at (m0, n0) over k in [kbeg, kend), by one workgroup of
-// TL::NT threads; smem holds 4 x TL::TU4 slots ([stage][A|B]). EPI_PARTIAL writes
-// slab `slab` of the split-K partials. TRANS bit 0: A is stored transposed (K x M,
-// op(A) = A^T); bit 1: B is stored transposed (N x K). FP32 kinds only.
-template <int KIND, int EPI, class TL, int TRANS>
+// KG x TL::NT threads; smem holds KG x 4 x TL::TU4 slots ([group][stage][A|B]).
+// EPI_PARTIAL writes slab `slab` of the split-K partials. TRANS bit 0: A is stored
+// transposed (K x M, op(A) = A^T); bit 1: B is stored transposed (N x K). FP32 kinds
+// only.
+// KG = 2 (FP32 kinds, C stored): two groups of TL::NW waves share the tile, group g
+// taking the g-th half of its k stages, so a CU holding one tile still has two waves
+// per SIMD; group 1 hands its accumulators to group 0 through LDS, which adds them
+// (first half + second half, a fixed order) and stores C.
+template <int KIND, int EPI, class TL, int TRANS, int KG = 1>
 __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const void *__restrict__ Bv,
                                           void *__restrict__ Cv, const int *__restrict__ colsum, long M, long N,
                                           long K, long kbeg, long kend, long m0, long n0, int slab, int vecA,
-                                          int vecB, uint4 *smem) {
+                                          int vecB, uint4 *smem_all) {
   using T = KT<KIND>;
   using Acc = typename T::Acc;
   constexpr int BT = TL::BT, IM = TL::IM, JN = TL::JN, TILE_U4 = TL::TU4;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  static_assert(KG == 1 || ((KIND == K_F32 || KIND == K_F32X3) && EPI == EPI_STORE), "k groups: FP32 kinds, C");
+  const int grp = KG == 1 ? 0 : (int)(threadIdx.x / TL::NT);
+  const int t = KG == 1 ? (int)threadIdx.x : (int)(threadIdx.x - grp * TL::NT), lane = t & 63, wave = t >> 6;
   const int wm = wave / TL::WN, wn = wave % TL::WN;
+  uint4 *const smem = smem_all + grp * 4 * TILE_U4;
 
   Acc acc[IM][JN];
 #pragma unroll
@@ -388,21 +400,49 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
     if (TB) st.store_rows(dst + TILE_U4, t, st.b);
     else st.store_cols(dst + TILE_U4, t, cs, st.b);
   };
-  int nstages = (int)((kend - kbeg + T::BK - 1) / T::BK);
+  const int total = (int)((kend - kbeg + T::BK - 1) / T::BK);
+  // stages of this group: [first, first + nstages); every group runs `iters` barriers
+  const int iters = KG == 1 ? total : (total + 1) / 2;
+  const int first = grp * iters;
+  const int nstages = KG == 1 ? total : (grp == 0 ? iters : total - iters);
+  const long gk = kbeg + (long)first * T::BK;
   if (nstages > 0) {
-    load(kbeg);
+    load(gk);
     store(smem);
   }
   __syncthreads();
-  for (int s = 0; s < nstages; ++s) {
+  for (int s = 0; s < iters; ++s) {
     uint4 *cur = smem + (s & 1) * 2 * TILE_U4;
     uint4 *nxt = smem + ((s + 1) & 1) * 2 * TILE_U4;
     const bool more = s + 1 < nstages;
-    if (more) load(kbeg + (long)(s + 1) * T::BK);
+    if (more) load(gk + (long)(s + 1) * T::BK);
+    if (KG == 1 || s < nstages) {
 #pragma unroll
-    for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND, TL>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
+      for (int kc = 0; kc < T::BK; kc += T::KC) mma_step<KIND, TL>(cur, cur + TILE_U4, kc, wm, wn, lane, acc);
+    }
     if (more) store(nxt);
     __syncthreads();
+  }
+  if (KG == 2) {
+    // group 1's accumulators -> LDS (lane-contiguous), group 0 adds them
+    float *xs = (float *)smem_all;
+    constexpr int PER_WAVE = IM * JN * 16 * 64;
+    if (grp == 1) {
+#pragma unroll
+      for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+#pragma unroll
+          for (int r = 0; r < 16; ++r) xs[wave * PER_WAVE + ((i * JN + j) * 16 + r) * 64 + lane] = (float)acc[i][j][r];
+    }
+    __syncthreads();
+    if (grp == 1) return;
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][j][r] += xs[wave * PER_WAVE + ((i * JN + j) * 16 + r) * 64 + lane];
   }
   // INT8 without a column-sum array (no split-K): the workgroup staged every row of its
   // B panel, so it sums the panel's columns itself (threads of one column block added
@@ -479,18 +519,18 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
   }
 }
 
-template <int KIND, int EPI, class TL, int TRANS>
-__global__ __launch_bounds__(TL::NT) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
-                                                      void *__restrict__ Cv, const int *__restrict__ colsum,
-                                                      long M, long N, long K, long kper, int tiles_m, int tiles_n,
-                                                      int vecA, int vecB) {
+template <int KIND, int EPI, class TL, int TRANS, int KG = 1>
+__global__ __launch_bounds__(TL::NT * KG) void gemm_kernel(const void *__restrict__ Av, const void *__restrict__ Bv,
+                                                           void *__restrict__ Cv, const int *__restrict__ colsum,
+                                                           long M, long N, long K, long kper, int tiles_m,
+                                                           int tiles_n, int vecA, int vecB) {
   extern __shared__ uint4 smem[];
   int tm, tn;
   tile_of(blockIdx.x, tiles_m, tiles_n, tm, tn);
   const long kbeg = (long)blockIdx.y * kper;
   const long kend = (kbeg + kper < K) ? kbeg + kper : K;
-  gemm_tile<KIND, EPI, TL, TRANS>(Av, Bv, Cv, colsum, M, N, K, kbeg, kend, (long)tm * TL::BT, (long)tn * TL::BT,
-                                  (int)blockIdx.y, vecA, vecB, smem);
+  gemm_tile<KIND, EPI, TL, TRANS, KG>(Av, Bv, Cv, colsum, M, N, K, kbeg, kend, (long)tm * TL::BT,
+                                      (long)tn * TL::BT, (int)blockIdx.y, vecA, vecB, smem);
 }
 
 // Grouped launch over a LevelTable: workgroup b runs item b, one 128 x 128 tile of one
@@ -650,7 +690,7 @@ __global__ void ref_permute_kernel(const E *__restrict__ Z, E *__restrict__ X, l
   X[e] = Z[(size_t)r * p.N + p0 + c];
 }
 
-template <int KIND, int EPI, class TL, int TRANS>
+template <int KIND, int EPI, class TL, int TRANS, int KG = 1>
 hipError_t launch_tile(const GemmArgs &g, hipStream_t s) {
   using T = KT<KIND>;
   int tiles_m = (int)((g.M + TL::BT - 1) / TL::BT), tiles_n = (int)((g.N + TL::BT - 1) / TL::BT);
@@ -665,22 +705,29 @@ hipError_t launch_tile(const GemmArgs &g, hipStream_t s) {
   };
   bool vecA = (TRANS & 1) ? vec_cols(g.A, g.M) : vec_rows(g.A, g.K);
   bool vecB = (TRANS & 2) ? vec_rows(g.B, g.K) : vec_cols(g.B, g.N);
-  const size_t lds = 4 * (size_t)TL::TU4 * sizeof(uint4);
-  auto kfn = gemm_kernel<KIND, EPI, TL, TRANS>;
+  const size_t lds = KG * 4 * (size_t)TL::TU4 * sizeof(uint4);
+  auto kfn = gemm_kernel<KIND, EPI, TL, TRANS, KG>;
   hipError_t e = hipFuncSetAttribute((const void *)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
   dim3 grid(tiles_m * tiles_n, splits);
-  hipLaunchKernelGGL(kfn, grid, dim3(TL::NT), lds, s, g.A, g.B, g.C, g.colsum, g.M, g.N, g.K, kper, tiles_m, tiles_n,
-                     (int)vecA, (int)vecB);
+  hipLaunchKernelGGL(kfn, grid, dim3(TL::NT * KG), lds, s, g.A, g.B, g.C, g.colsum, g.M, g.N, g.K, kper, tiles_m,
+                     tiles_n, (int)vecA, (int)vecB);
   return hipGetLastError();
 }
 
 // The 256 x 256 tile reads half the operand bytes per MFMA of the 128 x 128 tile;
 // it is used when it still gives about one tile per CU.
+// A single FP32 GEMM (g.kgroups) with at most one 128 x 128 tile per CU and a long K
+// runs two k groups per tile (8 waves), for two waves per SIMD.
 template <int KIND, int EPI, int TRANS>
 hipError_t launch_trans(const GemmArgs &g, hipStream_t s) {
   const long big = ((g.M + 255) / 256) * ((g.N + 255) / 256) * (g.splits > 1 ? g.splits : 1);
   if (big >= 192) return launch_tile<KIND, EPI, TileL, TRANS>(g, s);
+  if constexpr ((KIND == K_F32 || KIND == K_F32X3) && EPI == EPI_STORE) {
+    const long small = ((g.M + 127) / 128) * ((g.N + 127) / 128);
+    if (g.kgroups && g.splits <= 1 && small <= num_cus() && g.K >= 16 * KT<KIND>::BK)
+      return launch_tile<KIND, EPI, TileS, TRANS, 2>(g, s);
+  }
   return launch_tile<KIND, EPI, TileS, TRANS>(g, s);
 }
 

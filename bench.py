@@ -12,8 +12,9 @@ Also reported (``configs``), each at its full size:
   * skinny FP32 GEMMs M=8192,N=64,K=8192 and M=N=64,K=65536 (split-K, HBM-bound);
   * the INT8 and BF16 3-chains X(16384x512).W1(512x512).W2(512x1024).W3(1024x512);
   * the FP32 precision study (4-chain, 16384 x 1024->4096->1024->4096->1024, columns
-    scaled log-uniform in [1e-3, 1e3]): plain FP32 MFMA vs split-bf16 x3, relative
-    Frobenius error of each against the FP64 oracle on the GPU;
+    scaled log-uniform in [1e-3, 1e3]): plain FP32 MFMA vs split-bf16 x3 (hi + lo) and
+    the three-piece split x6 (hi + mid + lo), relative Frobenius error of each against
+    the FP64 oracle on the GPU;
   * serving: 10,000 Poisson chain requests against 64 resident weight sets, p50/p99
     latency and throughput of the engine (continuous batching, grouped kernel under a
     hipGraph) vs the same trace with one launch per GEMM.
@@ -359,14 +360,16 @@ def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
     fg.f64_chain(ref, Bd, Ad, N=N, Ms=Ms, Ks=Ks, stream=stream)
     out = {}
     flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
-    for name, mode in (("fp32_mfma", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3)):
+    for name, mode in (("fp32_mfma", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3), ("bf16x6", fg.F32_BF16X6)):
         C = torch.empty((widths[-1], N), device="cuda", dtype=torch.float32)
         o = fg.opts(f32_mode=mode)
         ms = timer.run(lambda: fg.chain(fg.DTYPE_F32, C, B, As, o=o, stream=stream), steps, warmup)
         d2, r2 = fg.error_norms(0, C, ref, C.numel(), stream=stream)
         out[name] = dict(ms=ms, tflops=flop / ms / 1e9, rel_err_vs_fp64=(d2 / r2) ** 0.5)
-    out["speedup_bf16x3_over_fp32"] = out["fp32_mfma"]["ms"] / out["bf16x3"]["ms"]
-    out["err_ratio_bf16x3_over_fp32"] = out["bf16x3"]["rel_err_vs_fp64"] / max(out["fp32_mfma"]["rel_err_vs_fp64"], 1e-30)
+    e32 = max(out["fp32_mfma"]["rel_err_vs_fp64"], 1e-30)
+    for name in ("bf16x3", "bf16x6"):  # two-piece and three-piece split
+        out[f"speedup_{name}_over_fp32"] = out["fp32_mfma"]["ms"] / out[name]["ms"]
+        out[f"err_ratio_{name}_over_fp32"] = out[name]["rel_err_vs_fp64"] / e32
     return out
 
 

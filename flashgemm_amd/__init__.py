@@ -25,7 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libflashgemm.so")
 
 DTYPE_F32, DTYPE_BF16, DTYPE_INT8 = 0, 1, 2
-F32_MFMA, F32_BF16X3 = 0, 1
+F32_MFMA, F32_BF16X3, F32_BF16X6 = 0, 1, 2
 LAYOUT_MATH, LAYOUT_REFERENCE = 0, 1
 FUSE_AUTO, FUSE_OFF, FUSE_ON = -1, 0, 1
 (OK, ERR_INVALID, ERR_DOMAIN, ERR_NOMEM, ERR_HIP, ERR_BUDGET, ERR_NODEVICE,

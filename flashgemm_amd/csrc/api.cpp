@@ -102,7 +102,12 @@ size_t dtype_bytes(int dtype) { return dtype == FG_DTYPE_F32 ? 4 : dtype == FG_D
 int kind_of(int dtype, int f32_mode) {
   if (dtype == FG_DTYPE_BF16) return K_BF16;
   if (dtype == FG_DTYPE_INT8) return K_I8;
-  return f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  return f32_mode == FG_F32_BF16X3 ? K_F32X3 : (f32_mode == FG_F32_BF16X6 ? K_F32X6 : K_F32);
+}
+
+int check_f32_mode(int f32_mode) {
+  if (f32_mode < FG_F32_MFMA || f32_mode > FG_F32_BF16X6) return set_error(FG_ERR_INVALID, "f32_mode %d", f32_mode);
+  return FG_OK;
 }
 
 // ----------------------------------------------------------------- planner --
@@ -294,6 +299,7 @@ void fg_chain_opts_default(fg_chain_opts *o) {
 
 size_t fg_packed_weight_bytes(int dtype, int f32_mode, long M, long K) {
   if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8 || M <= 0 || K <= 0) return 0;
+  if (kind_of(dtype, f32_mode) == K_F32X6) return 0;  // unfused only: no packed weights
   return packed_a_bytes(kind_of(dtype, f32_mode), M, K);
 }
 
@@ -302,6 +308,9 @@ int fg_pack_weight(int dtype, int f32_mode, const void *A, long M, long K, void 
   if ((rc = ensure_init())) return rc;
   if (dtype < FG_DTYPE_F32 || dtype > FG_DTYPE_INT8 || !A || !packed || M <= 0 || K <= 0)
     return set_error(FG_ERR_INVALID, "pack_weight: dtype=%d M=%ld K=%ld", dtype, M, K);
+  if ((rc = check_f32_mode(f32_mode))) return rc;
+  if (kind_of(dtype, f32_mode) == K_F32X6)
+    return set_error(FG_ERR_UNSUPPORTED, "the three-piece split runs unfused: no packed weights");
   PackJob J{};
   J.n = 1;
   J.A[0] = A;
@@ -403,6 +412,17 @@ static int plan_chain(int dtype, long N, int num_gemm, const long *M, const long
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
   int kind = kind_of(dtype, o.f32_mode);
+  if (kind == K_F32X6) {  // the three-piece split runs unfused, one BF16 GEMM per chain GEMM
+    if (o.fuse == FG_FUSE_ON || o.tile_n > 0) return set_error(FG_ERR_UNSUPPORTED, "the three-piece split runs unfused");
+    if (refl_out) *refl_out = 0;
+    pl->splits = o.splits;
+    pl->nseg = num_gemm;
+    for (int i = 0; i < num_gemm; ++i) {
+      pl->end[i] = i;
+      pl->tile[i] = 0;
+    }
+    return FG_OK;
+  }
   const bool any_ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
   bool refl = false;
   if (any_ref && !epi && !img) {
@@ -494,6 +514,17 @@ static int run_gemm(int kind, int epi, const void *A, const void *B, void *C, lo
                     size_t ws_bytes = 0, int kgroups = 0) {
   int *colsum = nullptr;
   int rc;
+  if (kind == K_F32X6) {
+    // three-piece split: one BF16 GEMM over k' = 6K on the split operands, fp32 out
+    // (the FP32 kinds' requantization keeps fp32)
+    if (trans) return set_error(FG_ERR_UNSUPPORTED, "the three-piece split takes NN operands only");
+    if (ws) return set_error(FG_ERR_UNSUPPORTED, "the three-piece split needs scratch for its split operands");
+    uint16_t *A6 = nullptr, *B6 = nullptr;
+    if ((rc = scratch(s, "x6a", (size_t)M * 6 * K * 2, (void **)&A6))) return rc;
+    if ((rc = scratch(s, "x6b", (size_t)6 * K * N * 2, (void **)&B6))) return rc;
+    if ((rc = hip_status(launch_split3((const float *)A, M, K, A6, (const float *)B, N, B6, s), "split3"))) return rc;
+    return run_gemm(K_BF16, epi == EPI_REQUANT ? EPI_STORE : epi, A6, B6, C, M, N, 6 * K, splits, s, ex);
+  }
   auto partials = [&](size_t bytes, void **p) -> int {
     if (!ws) return scratch(s, "partials", bytes, p);
     if (bytes > ws_bytes)
@@ -672,6 +703,7 @@ static int chain_impl(int dtype, void *C, const void *B, long N, int num_gemm, c
   if (!B || !C) return set_error(FG_ERR_INVALID, "B or C is NULL");
   fg_chain_opts o;
   if (opts) o = *opts; else fg_chain_opts_default(&o);
+  if ((rc = check_f32_mode(o.f32_mode))) return rc;
   hipStream_t s = (hipStream_t)stream;
   int kind = kind_of(dtype, o.f32_mode);
   bool ref = o.layout == FG_LAYOUT_REFERENCE && dtype != FG_DTYPE_F32;
@@ -739,7 +771,8 @@ int fg_sgemm(float *C, const float *A, const float *B, long M, long N, long K, i
   if ((rc = ensure_init())) return rc;
   if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
-  int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  if ((rc = check_f32_mode(f32_mode))) return rc;
+  const int kind = kind_of(FG_DTYPE_F32, f32_mode);
   return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, 0, nullptr, 0, 1);
 }
 
@@ -754,7 +787,8 @@ int sgemm_ws(float *C, const float *A, const float *B, long M, long N, long K, i
   if ((rc = ensure_init())) return rc;
   if (!A || !B || !C || M <= 0 || N <= 0 || K <= 0)
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
-  const int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  if ((rc = check_f32_mode(f32_mode))) return rc;
+  const int kind = kind_of(FG_DTYPE_F32, f32_mode);
   return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, 0, ws, ws_bytes, 1);
 }
 }  // namespace fg
@@ -769,7 +803,8 @@ int fg_sgemm_t(int transa, int transb, float *C, const float *A, const float *B,
     return set_error(FG_ERR_INVALID, "sgemm: M=%ld N=%ld K=%ld", M, N, K);
   if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1))
     return set_error(FG_ERR_INVALID, "sgemm: transa=%d transb=%d (0 or 1)", transa, transb);
-  int kind = f32_mode == FG_F32_BF16X3 ? K_F32X3 : K_F32;
+  if ((rc = check_f32_mode(f32_mode))) return rc;
+  const int kind = kind_of(FG_DTYPE_F32, f32_mode);
   return run_gemm(kind, EPI_STORE, A, B, C, M, N, K, splits, (hipStream_t)stream, nullptr, transa | (transb << 1),
                   nullptr, 0, 1);
 }

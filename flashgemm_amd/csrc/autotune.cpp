@@ -156,7 +156,9 @@ int fg_autotune_chain(int dtype, int f32_mode, long N, int num_gemm, const long 
     if (M[i] <= 0 || K[i] <= 0 || (i > 0 && K[i] != M[i - 1])) return set_error(FG_ERR_INVALID, "autotune shapes");
   hipStream_t s = (hipStream_t)stream;
   const int esz = dtype == FG_DTYPE_F32 ? 4 : dtype == FG_DTYPE_BF16 ? 2 : 1;
+  if ((rc = check_f32_mode(f32_mode))) return rc;
   const int kind = kind_of(dtype, f32_mode);
+  if (kind == K_F32X6) return set_error(FG_ERR_UNSUPPORTED, "the three-piece split has one plan (unfused)");
   const std::vector<ChainPlan> cands = candidates(kind, N, num_gemm, M, K);
   if (cands.empty()) return set_error(FG_ERR_UNSUPPORTED, "no runnable plan");
   // operands: seeded random data of the chain's dtype (timing only)

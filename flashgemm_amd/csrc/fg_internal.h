@@ -29,7 +29,9 @@ enum Epilogue : int {
 };
 
 // Compute "kinds" = (operand storage, MFMA) combinations.
-enum Kind : int { K_F32 = 0, K_BF16 = 1, K_I8 = 2, K_F32X3 = 3 };
+// K_F32X6 (three-piece split FP32) is host-side only: it runs as a BF16 GEMM over the
+// split operands (launch_split3), never as a kernel kind.
+enum Kind : int { K_F32 = 0, K_BF16 = 1, K_I8 = 2, K_F32X3 = 3, K_F32X6 = 4 };
 
 typedef __attribute__((ext_vector_type(16))) float f32x16;
 typedef __attribute__((ext_vector_type(16))) int i32x16;
@@ -200,6 +202,14 @@ int skinny_f32_splits(long M, long N, long K, long *kper);
 hipError_t launch_skinny_f32(const float *A, const float *B, float *C, long M, long N, long K, long kper,
                              int splits, hipStream_t s);
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s);
+// Three-piece split of FP32 operands for C = A.B as one BF16 GEMM over k' = 6K:
+// x = hi + mid + lo (each bf16 round-to-nearest-even of the remainder; the remainders
+// are exact in fp32); A6 (M x 6K) has column blocks [hi hi mid hi lo mid] of A and
+// B6 (6K x N) row blocks [hi mid hi lo hi mid] of B: block j of A6 meets block j of B6, so
+// the k' sum is hi.hi + hi.mid + mid.hi + hi.lo + lo.hi + mid.mid (the products whose
+// pieces sum to order <= 2; mid.lo, lo.mid, lo.lo are dropped, below 2^-24 relative).
+hipError_t launch_split3(const float *A, long M, long K, uint16_t *A6, const float *B, long N, uint16_t *B6,
+                         hipStream_t s);
 // Sums the slabs and applies the epilogue `epi`, with the extensions of `ex` when it
 // is not null (launch_splitk_reduce is also the epilogue pass of unfused GEMMs that
 // have extensions).

@@ -46,6 +46,8 @@ size_t chain_scratch_bound(int dtype, int f32_mode, long N, int n, const long *M
 int sgemm_ws(float *C, const float *A, const float *B, long M, long N, long K, int f32_mode, int splits, void *stream,
              void *ws, size_t ws_bytes);
 int kind_of(int dtype, int f32_mode);
+// FG_ERR_INVALID (with the message set) unless f32_mode is one of FG_F32_*.
+int check_f32_mode(int f32_mode);
 int plan_splits(long M, long N, long K, int kind);
 int plan_tile_n(int kind, long N, int num_gemm, const long *M, const long *K, bool epi = false);
 

@@ -805,6 +805,45 @@ hipError_t launch_grouped_gemm(const void *table, int grid, hipStream_t s) {
   return hipGetLastError();
 }
 
+__device__ __forceinline__ void split3(float x, uint16_t &h, uint16_t &m, uint16_t &l) {
+  h = bf16_dev(x);
+  const float r1 = x - __builtin_bit_cast(float, (uint32_t)h << 16);
+  m = bf16_dev(r1);
+  l = bf16_dev(r1 - __builtin_bit_cast(float, (uint32_t)m << 16));
+}
+
+// A (M x K) -> A6 (M x 6K), column blocks [hi hi mid hi lo mid]; one element per thread.
+__global__ __launch_bounds__(256) void split3_a_kernel(const float *__restrict__ A, long M, long K,
+                                                       uint16_t *__restrict__ A6) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= M * K) return;
+  const long m = e / K, k = e - m * K;
+  uint16_t h, md, l;
+  split3(A[e], h, md, l);
+  uint16_t *row = A6 + m * 6 * K + k;
+  row[0] = h; row[K] = h; row[2 * K] = md; row[3 * K] = h; row[4 * K] = l; row[5 * K] = md;
+}
+
+// B (K x N) -> B6 (6K x N), row blocks [hi mid hi lo hi mid].
+__global__ __launch_bounds__(256) void split3_b_kernel(const float *__restrict__ B, long K, long N,
+                                                       uint16_t *__restrict__ B6) {
+  const long e = (long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= K * N) return;
+  uint16_t h, md, l;
+  split3(B[e], h, md, l);
+  const long blk = K * N;
+  B6[e] = h; B6[blk + e] = md; B6[2 * blk + e] = h; B6[3 * blk + e] = l; B6[4 * blk + e] = h; B6[5 * blk + e] = md;
+}
+
+hipError_t launch_split3(const float *A, long M, long K, uint16_t *A6, const float *B, long N, uint16_t *B6,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(split3_a_kernel, dim3((unsigned)((M * K + 255) / 256)), dim3(256), 0, s, A, M, K, A6);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(split3_b_kernel, dim3((unsigned)((K * N + 255) / 256)), dim3(256), 0, s, B, K, N, B6);
+  return hipGetLastError();
+}
+
 hipError_t launch_colsum_i8(const int8_t *B, long K, long N, int *out, hipStream_t s) {
   hipLaunchKernelGGL(colsum_zero_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, out, N);
   dim3 grid((unsigned)((N + 255) / 256), (unsigned)((K + CS_ROWS - 1) / CS_ROWS));

@@ -41,7 +41,9 @@ int describe(const fg_chain_req &r, ChainParams *P, int *tile_n, size_t *lds, bo
     if (!r.A[i] || r.M[i] <= 0 || r.K[i] <= 0) return set_error(FG_ERR_INVALID, "request gemm %d", i + 1);
     if (i > 0 && r.K[i] != r.M[i - 1]) return set_error(FG_ERR_INVALID, "request gemm %d: K != previous M", i + 1);
   }
+  if (int rc = check_f32_mode(r.f32_mode)) return rc;
   int kind = kind_of(r.dtype, r.f32_mode);
+  if (kind == K_F32X6) return set_error(FG_ERR_UNSUPPORTED, "the three-piece split runs through fg_chain only");
   ChainArgs c{};
   c.kind = kind;
   c.num_gemm = r.num_gemm;
@@ -310,6 +312,9 @@ int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream) {
     if (q.num_gemm < 1 || q.num_gemm > CHAIN_MAXG || q.N <= 0 || !q.B || !q.C)
       return set_error(FG_ERR_INVALID, "request %d: num_gemm=%d N=%ld", r, q.num_gemm, q.N);
     if (q.a_packed) return set_error(FG_ERR_UNSUPPORTED, "request %d: packed weights need the fused path", r);
+    if ((rc = check_f32_mode(q.f32_mode))) return rc;
+    if (kind_of(q.dtype, q.f32_mode) == K_F32X6)
+      return set_error(FG_ERR_UNSUPPORTED, "request %d: the three-piece split runs through fg_chain only", r);
     long maxm = 0;
     for (int i = 0; i < q.num_gemm; ++i) {
       if (!q.A[i] || q.M[i] <= 0 || q.K[i] <= 0) return set_error(FG_ERR_INVALID, "request %d gemm %d", r, i + 1);

@@ -73,14 +73,18 @@ enum fg_status {
 };
 
 enum fg_dtype { FG_DTYPE_F32 = 0, FG_DTYPE_BF16 = 1, FG_DTYPE_INT8 = 2 };
-enum fg_f32_mode { FG_F32_MFMA = 0, FG_F32_BF16X3 = 1 };
+/* FP32 arithmetic: FG_F32_MFMA (fp32 MFMA), FG_F32_BF16X3 (two-piece split,
+ * x = hi + lo in bf16, three bf16 products per pair, fp32 accumulation), or
+ * FG_F32_BF16X6 (three-piece split x = hi + mid + lo, the six products of order <= 2:
+ * close to FP32 MFMA error; unfused, NN operands; runs as one BF16 GEMM over 6K). */
+enum fg_f32_mode { FG_F32_MFMA = 0, FG_F32_BF16X3 = 1, FG_F32_BF16X6 = 2 };
 enum fg_layout { FG_LAYOUT_MATH = 0, FG_LAYOUT_REFERENCE = 1 };
 enum fg_fuse { FG_FUSE_AUTO = -1, FG_FUSE_OFF = 0, FG_FUSE_ON = 1 };
 
 typedef struct fg_chain_opts {
   int layout;      /* FG_LAYOUT_MATH (default) or FG_LAYOUT_REFERENCE */
   int ref_threads; /* thread count the REFERENCE layout follows (>= 1)   */
-  int f32_mode;    /* FG_F32_MFMA (default) or FG_F32_BF16X3            */
+  int f32_mode;    /* FG_F32_MFMA (default), FG_F32_BF16X3, FG_F32_BF16X6 */
   int fuse;        /* FG_FUSE_AUTO (default), FG_FUSE_OFF, FG_FUSE_ON    */
   int tile_n;      /* fused tile width: 0 = planner, or 32, 64, 96, 128, 256 (96 not for INT8/BF16) */
   int splits;      /* split-K of unfused GEMMs: 0 = planner              */

@@ -197,8 +197,9 @@ def test_bf16_chain3_full_size(gpu):
 # ------------------------------------------------- FP32 precision 4-chain --
 
 def test_precision_chain_full_size_and_f64_oracle(gpu):
-    """The precision study at its shapes and scaled data: plain FP32 and split-bf16 x3
-    against the CPU FP64 chain on sampled columns; the GPU FP64 chain (the reference
+    """The precision study at its shapes and scaled data: plain FP32, split-bf16 x3 and
+    the three-piece split (x6, its error at most twice FP32's) against the CPU FP64
+    chain on sampled columns; the GPU FP64 chain (the reference
     every precision figure is measured against) against the CPU FP64 chain; and
     fg_error_norms against numpy over the whole output."""
     import torch
@@ -210,7 +211,7 @@ def test_precision_chain_full_size_and_f64_oracle(gpu):
     Bd, Ad = B.double(), [a.double() for a in As]  # alive until the queued chain has run
     fg.f64_chain(ref, Bd, Ad)
     outs = {}
-    for name, mode in (("fp32", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3)):
+    for name, mode in (("fp32", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3), ("bf16x6", fg.F32_BF16X6)):
         C = empty((Ms[-1], N), torch.float32)
         fg.chain(fg.DTYPE_F32, C, B, As, o=fg.opts(f32_mode=mode))
         outs[name] = C
@@ -221,6 +222,8 @@ def test_precision_chain_full_size_and_f64_oracle(gpu):
     report("GPU fp64 chain vs CPU fp64 chain", O.rel_fro(host(ref)[:, cols], want), TOL_F64_ORACLE)
     report("fp32 rel vs fp64", O.rel_fro(host(outs["fp32"])[:, cols], want), TOL_F32)
     report("bf16x3 rel vs fp64", O.rel_fro(host(outs["bf16x3"])[:, cols], want), TOL_X3)
+    e32 = O.rel_fro(host(outs["fp32"])[:, cols], want)
+    report("bf16x6 rel vs fp64", O.rel_fro(host(outs["bf16x6"])[:, cols], want), 2 * e32)
     refh = host(ref)
     for name, C in outs.items():
         d2, r2 = fg.error_norms(0, C, ref, C.numel())

@@ -13,6 +13,17 @@ reference's kernels leave rows unwritten or read misaligned data, measured as:
   * INT8: every M_i % 12 == 0 (rows past the last full 12-row block are wrong);
   * BF16 with a 16-wide panel: first GEMM M_1 <= 20 and later M_i % 12 == 0;
   * singlegemm: K % 2048 == 8 crashes the reference.
+
+The INT8 bound is also a CPU test (tests/test_reference_domain.py, run when oracle/_ref
+is built).
+
+The fixtures depend on the order of the calls below: the reference's singlegemm
+keeps its block sizes in process globals that earlier calls set
+(singlegemm/src/set_TmTn.h:25-37). ``sgemm_20x16x2336_ktail`` is what the reference
+computes after the calls before it in this script; issued first in a fresh process,
+the same call crashes the reference. The fixture stays valid as data: the oracle
+restates the arithmetic the reference performs in this order (2048-K blocks added in
+order, then the K tail), and tests compare the oracle with the saved C.
 """
 from __future__ import annotations
 

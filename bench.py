@@ -482,6 +482,10 @@ def main() -> None:
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run 5 device copies of 256 MiB (reads 256 MiB, writes 256 MiB each) and exit: "
                          "the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--clock-calibration", action="store_true",
+                    help="run the FP32 2-chain and the INT8 / BF16 3-chains 20 times each on the benchmark data "
+                         "and 20 times on an all-zero B, and exit: under a GRBM_GUI_ACTIVE counter run the same "
+                         "cycle counts at different durations show the clock's dependence on the data")
     ap.add_argument("--cpu-baseline-child", nargs=2, metavar=("INPUTS", "SECONDS"), default=None,
                     help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -490,7 +494,7 @@ def main() -> None:
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "design-only, see DESIGN.md)"}))
+                                   "a design, not built)"}))
         sys.exit(2)
 
     import torch
@@ -509,6 +513,17 @@ def main() -> None:
     timer = Timer(torch, stream)
     if args.stream_calibration:
         print(json.dumps({"stream_calibration": stream_calibration(fg, stream)}))
+        return
+    if args.clock_calibration:
+        out = {}
+        for dt in (fg.DTYPE_F32, fg.DTYPE_INT8, fg.DTYPE_BF16):
+            B, As = W.chain2_f32(torch) if dt == fg.DTYPE_F32 else W.chain3(torch, dt)
+            C = torch.empty((As[-1].shape[0], B.shape[1]), device="cuda",
+                            dtype=torch.int32 if dt == fg.DTYPE_INT8 else torch.float32)
+            for name, b in (("data", B), ("zero_b", torch.zeros_like(B))):
+                ms = timer.run(lambda: fg.chain(dt, C, b, As, stream=stream), 20, 0)
+                out[f"dtype{dt}_{name}_ms"] = ms
+        print(json.dumps({"clock_calibration": out}))
         return
 
     extra = (

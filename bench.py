@@ -317,9 +317,8 @@ def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0, 40000.0), seed
 
 def verify_serving(torch, fg, sets, set_of, Ns, sample, kept, refs):
     """Sampled requests of a serving run: the engine's result equals the same request
-    launched alone through fg_chain with the engine's plan for it (default options
-    when it fits the on-chip grouped tile, else unfused without split-K: the level
-    launch) bit for bit; INT8 results of the engine and of the per-GEMM baseline are
+    launched alone with the engine's plan for it (fg_chain when it fits the on-chip
+    grouped tile, else the level launch) bit for bit; INT8 results of the engine and of the per-GEMM baseline are
     equal; FP32 / BF16 relative Frobenius error against FP64 per path and dtype (the
     p99 comparison is at these accuracies)."""
     alone_same, int8_same = 0, 0
@@ -330,9 +329,11 @@ def verify_serving(torch, fg, sets, set_of, Ns, sample, kept, refs):
         s, n = sets[set_of[i]], int(Ns[i])
         w = s["w"]
         single = torch.empty_like(kept["engine"][i])
-        fits = fg.grouped_fits(fg.chain_req(s["dtype"], single, s["B"], s["As"], N=n, Ms=w[1:], Ks=w[:-1]))
-        o = None if fits else fg.opts(fuse=fg.FUSE_OFF, splits=1)
-        fg.chain(s["dtype"], single, s["B"], s["As"], N=n, Ms=w[1:], Ks=w[:-1], o=o)
+        req = fg.chain_req(s["dtype"], single, s["B"], s["As"], N=n, Ms=w[1:], Ks=w[:-1])
+        if fg.grouped_fits(req):
+            fg.chain(s["dtype"], single, s["B"], s["As"], N=n, Ms=w[1:], Ks=w[:-1])
+        else:
+            fg.grouped_levels([req])
         torch.cuda.synchronize()
         alone_same += int(torch.equal(single, kept["engine"][i]))
         if s["dtype"] == fg.DTYPE_INT8:

@@ -331,15 +331,18 @@ hipError_t launch_grouped(const void *table, int grid, size_t lds, bool wide, hi
 
 // Grouped unfused GEMMs: one launch runs the 128 x 128 output tiles of many
 // independent GEMMs (one level of many chains). Table: LevelTable header, GemmDesc[]
-// at descs_off, int2 {desc, tile} items at items_off; tile t of a descriptor is
-// (t / tiles_n, t % tiles_n). No split-K: every tile runs the whole K, so a GEMM's
-// result has the bits of the same GEMM launched alone without split-K.
+// at descs_off, int2 {desc, tile * splits + slice} items at items_off; tile t of a
+// descriptor is (t / tiles_n, t % tiles_n). A descriptor with splits > 1 writes
+// EPI_PARTIAL slabs (slice s covers k [s * kper, (s + 1) * kper)) that a grouped
+// reduction (ReduceTable) adds in slab order; its split depends on its shape only, so
+// a GEMM's bits do not depend on the other GEMMs of the launch.
 struct GemmDesc {
   const void *A;  // M x K row-major
   const void *B;  // K x N row-major
-  void *C;        // M x N: EPI_STORE fp32 / int32, EPI_REQUANT the next operand
-  long M, N, K;
+  void *C;        // M x N: EPI_STORE fp32 / int32, EPI_REQUANT the next operand, EPI_PARTIAL slabs
+  long M, N, K, kper;
   int kind, epi, tiles_n, vec;  // vec: bit 0 A rows 16-byte aligned, bit 1 B columns vectorizable
+  int splits, pad;
 };
 struct LevelTable {
   int num_descs;
@@ -348,8 +351,27 @@ struct LevelTable {
   int items_off;  // bytes from the table start
 };
 constexpr int LEVEL_TILE = 128;
+// The reduction of a level's split GEMMs: item {desc, chunk} sums 256 consecutive
+// elements over the descriptor's slabs in slab order, then requantizes (EPI_REQUANT)
+// or stores (EPI_STORE) into C.
+struct ReduceDesc {
+  const void *P;  // splits slabs of MN 32-bit partials
+  void *C;
+  long MN;
+  int splits, kind, epi, pad;
+};
+struct ReduceTable {
+  int num_descs;
+  int num_items;
+  int descs_off;
+  int items_off;
+};
+hipError_t launch_grouped_reduce(const void *table, int grid, hipStream_t s);
 // Fills d (vector flags, tile counts) for C = A . B with the given kind and epilogue.
 void gemm_desc(GemmDesc *d, int kind, int epi, const void *A, const void *B, void *C, long M, long N, long K);
+// Splits d over k into about `splits` slices (whole k-stages; the count used is
+// returned, 1 = unsplit and d unchanged): d then writes EPI_PARTIAL slabs at `slabs`.
+int gemm_desc_split(GemmDesc *d, int splits, void *slabs);
 // One launch per level; each descriptor has its own kind and epilogue (EPI_STORE or
 // EPI_REQUANT). grid = the table's item count.
 hipError_t launch_grouped_gemm(const void *table, int grid, hipStream_t s);

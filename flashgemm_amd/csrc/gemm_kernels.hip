@@ -444,11 +444,13 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[i][j][r] += xs[wave * PER_WAVE + ((i * JN + j) * 16 + r) * 64 + lane];
   }
-  // INT8 without a column-sum array (no split-K): the workgroup staged every row of its
-  // B panel, so it sums the panel's columns itself (threads of one column block added
-  // in LDS, integers: exact in any order).
+  // INT8 without a column-sum array: the workgroup staged every row of its B panel
+  // (of its k-slice, for a split-K partial), so it sums the panel's columns itself
+  // (threads of one column block added in LDS, integers: exact in any order). A partial
+  // slab then carries its slice's share of the bias, and the slabs add up to the
+  // unsplit result exactly.
   const int *csum_tile = nullptr;
-  if (KIND == K_I8 && EPI != EPI_PARTIAL && colsum == nullptr) {
+  if (KIND == K_I8 && colsum == nullptr) {
     constexpr int NCB = BT / KT<K_I8>::CW, NG = TL::NT / NCB;
     int *red = (int *)smem;  // NG x BT partial sums, then BT totals
     const int cb = t % NCB, g = t / NCB;
@@ -472,7 +474,7 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
   constexpr int CE = (EPI == EPI_REQUANT && KIND == K_I8) ? 1 : ((EPI == EPI_REQUANT && KIND == K_BF16) ? 2 : 4);
   auto put = [&](char *p, const typename KT<KIND>::Acc &a, int r, int bias) {
     if (EPI == EPI_PARTIAL) {
-      if (KIND == K_I8) *(int *)p = (int)a[r];
+      if (KIND == K_I8) *(int *)p = (int)((unsigned)a[r] + (unsigned)bias);
       else *(float *)p = (float)a[r];
     } else if (KIND == K_I8) {
       const int v = (int)((unsigned)a[r] + (unsigned)bias);
@@ -492,7 +494,7 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
     for (int j = 0; j < JN; ++j) {
       const long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
       int bias = 0;
-      if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
+      if (KIND == K_I8 && (EPI != EPI_PARTIAL || !colsum)) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
 #pragma unroll
       for (int i = 0; i < IM; ++i) {
         char *const p = Cb + ((size_t)(m0 + wm * (BT / TL::WM) + i * 32 + 4 * h) * N + n) * CE;
@@ -507,7 +509,7 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
     long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
     if (n >= N) continue;
     int bias = 0;
-    if (KIND == K_I8 && EPI != EPI_PARTIAL) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
+    if (KIND == K_I8 && (EPI != EPI_PARTIAL || !colsum)) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
 #pragma unroll
     for (int i = 0; i < IM; ++i)
 #pragma unroll
@@ -534,9 +536,9 @@ __global__ __launch_bounds__(TL::NT * KG) void gemm_kernel(const void *__restric
 }
 
 // Grouped launch over a LevelTable: workgroup b runs item b, one 128 x 128 tile of one
-// GEMM over its whole K (the 256-thread tile of gemm_kernel, so a GEMM's bits match
-// its own unsplit launch), with the GEMM's kind and epilogue. One workgroup per item:
-// a persistent loop around the tile costs registers (INT8 to one wave per SIMD).
+// GEMM (or one k-slice of it, for a split descriptor) with the 256-thread tile of
+// gemm_kernel, with the GEMM's kind and epilogue. One workgroup per item: a persistent
+// loop around the tile costs registers (INT8 to one wave per SIMD).
 __global__ __launch_bounds__(TileS::NT) void grouped_gemm_kernel(const char *__restrict__ table) {
   extern __shared__ uint4 smem[];
   const LevelTable *hdr = (const LevelTable *)table;
@@ -544,31 +546,55 @@ __global__ __launch_bounds__(TileS::NT) void grouped_gemm_kernel(const char *__r
   const GemmDesc *descs = (const GemmDesc *)(table + hdr->descs_off);
   const int2 w = ((const int2 *)(table + hdr->items_off))[blockIdx.x];
   const GemmDesc &d = descs[w.x];
-  const int tm = w.y / d.tiles_n, tn = w.y - tm * d.tiles_n;
+  const int tile = w.y / d.splits, slice = w.y - tile * d.splits;
+  const int tm = tile / d.tiles_n, tn = tile - tm * d.tiles_n;
   const long m0 = (long)tm * TileS::BT, n0 = (long)tn * TileS::BT;
+  const long kb = (long)slice * d.kper, ke = kb + d.kper < d.K ? kb + d.kper : d.K;
   const int va = d.vec & 1, vb = (d.vec >> 1) & 1;
 #define FG_LEVEL_TILE(KIND, EPI) \
-  gemm_tile<KIND, EPI, TileS, 0>(d.A, d.B, d.C, nullptr, d.M, d.N, d.K, 0, d.K, m0, n0, 0, va, vb, smem)
-  const bool last = d.epi == EPI_STORE;
+  gemm_tile<KIND, EPI, TileS, 0>(d.A, d.B, d.C, nullptr, d.M, d.N, d.K, kb, ke, m0, n0, slice, va, vb, smem)
+#define FG_LEVEL_KIND(KIND)                                            \
+  do {                                                                 \
+    if (d.epi == EPI_STORE) FG_LEVEL_TILE(KIND, EPI_STORE);            \
+    else if (d.epi == EPI_REQUANT) FG_LEVEL_TILE(KIND, EPI_REQUANT);   \
+    else FG_LEVEL_TILE(KIND, EPI_PARTIAL);                             \
+  } while (0)
   switch (d.kind) {
-    case K_F32:
-      if (last) FG_LEVEL_TILE(K_F32, EPI_STORE);
-      else FG_LEVEL_TILE(K_F32, EPI_REQUANT);
-      break;
-    case K_BF16:
-      if (last) FG_LEVEL_TILE(K_BF16, EPI_STORE);
-      else FG_LEVEL_TILE(K_BF16, EPI_REQUANT);
-      break;
-    case K_I8:
-      if (last) FG_LEVEL_TILE(K_I8, EPI_STORE);
-      else FG_LEVEL_TILE(K_I8, EPI_REQUANT);
-      break;
-    case K_F32X3:
-      if (last) FG_LEVEL_TILE(K_F32X3, EPI_STORE);
-      else FG_LEVEL_TILE(K_F32X3, EPI_REQUANT);
-      break;
+    case K_F32: FG_LEVEL_KIND(K_F32); break;
+    case K_BF16: FG_LEVEL_KIND(K_BF16); break;
+    case K_I8: FG_LEVEL_KIND(K_I8); break;
+    case K_F32X3: FG_LEVEL_KIND(K_F32X3); break;
   }
+#undef FG_LEVEL_KIND
 #undef FG_LEVEL_TILE
+}
+
+// The split GEMMs of a level: workgroup b sums 256 elements of one descriptor over its
+// slabs, slab 0 first (a fixed order: the bits depend on the split, not on the launch),
+// and writes C in the descriptor's epilogue. INT8 slabs already hold their slices'
+// share of the u8 bias, so the integer sum is the unsplit result.
+__global__ __launch_bounds__(256) void grouped_reduce_kernel(const char *__restrict__ table) {
+  const ReduceTable *hdr = (const ReduceTable *)table;
+  if ((int)blockIdx.x >= hdr->num_items) return;
+  const ReduceDesc *descs = (const ReduceDesc *)(table + hdr->descs_off);
+  const int2 w = ((const int2 *)(table + hdr->items_off))[blockIdx.x];
+  const ReduceDesc &d = descs[w.x];
+  const size_t e = (size_t)w.y * 256 + threadIdx.x;
+  if (e >= (size_t)d.MN) return;
+  const size_t MN = (size_t)d.MN;
+  if (d.kind == K_I8) {
+    const unsigned *p = (const unsigned *)d.P + e;
+    unsigned s = 0;
+    for (int z = 0; z < d.splits; ++z) s += p[(size_t)z * MN];
+    if (d.epi == EPI_STORE) ((int *)d.C)[e] = (int)s;
+    else ((int8_t *)d.C)[e] = sat_i8((int)s);
+  } else {
+    const float *p = (const float *)d.P + e;
+    float s = p[0];
+    for (int z = 1; z < d.splits; ++z) s += p[(size_t)z * MN];
+    if (d.kind == K_BF16 && d.epi == EPI_REQUANT) ((uint16_t *)d.C)[e] = bf16_dev(s);
+    else ((float *)d.C)[e] = s;
+  }
 }
 
 // INT8 column sums sum_k s8(B[k][n]) for the u8 bias term (exact in int32 for
@@ -787,12 +813,34 @@ void gemm_desc(GemmDesc *d, int kind, int epi, const void *A, const void *B, voi
   d->kind = kind;
   d->epi = epi;
   d->tiles_n = (int)((N + TileS::BT - 1) / TileS::BT);
+  d->kper = K;
+  d->splits = 1;
+  d->pad = 0;
   switch (kind) {
     case K_F32: gemm_desc_kind<K_F32>(d, A, B, N, K); break;
     case K_BF16: gemm_desc_kind<K_BF16>(d, A, B, N, K); break;
     case K_I8: gemm_desc_kind<K_I8>(d, A, B, N, K); break;
     default: gemm_desc_kind<K_F32X3>(d, A, B, N, K); break;
   }
+}
+
+int gemm_desc_split(GemmDesc *d, int splits, void *slabs) {
+  const long bk = d->kind == K_I8 ? KT<K_I8>::BK : (d->kind == K_BF16 ? KT<K_BF16>::BK : KT<K_F32>::BK);
+  if (splits <= 1) return 1;
+  const long kper = ((d->K + splits - 1) / splits + bk - 1) / bk * bk;
+  const int used = (int)((d->K + kper - 1) / kper);
+  if (used <= 1) return 1;
+  d->kper = kper;
+  d->splits = used;
+  d->epi = EPI_PARTIAL;
+  d->C = slabs;
+  return used;
+}
+
+hipError_t launch_grouped_reduce(const void *table, int grid, hipStream_t s) {
+  if (grid <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(grouped_reduce_kernel, dim3(grid), dim3(256), 0, s, (const char *)table);
+  return hipGetLastError();
 }
 
 hipError_t launch_grouped_gemm(const void *table, int grid, hipStream_t s) {

@@ -330,20 +330,14 @@ int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream) {
   }
   char *spill = nullptr;
   if (total > 0 && (rc = scratch(s, "levels", total, (void **)&spill))) return rc;
-  // one table per level: descriptors with the longest tiles first (K over the kind's
-  // MFMA rate), then their tiles
-  auto tile_cost = [](const GemmDesc &d) {
-    const double rate = d.kind == K_I8 ? 32 : d.kind == K_BF16 ? 16 : d.kind == K_F32X3 ? 16.0 / 3 : 1;
-    return (double)d.K / rate;
-  };
-  struct Part {
-    int items;
-    size_t off;
-  };
-  std::vector<Part> parts;
-  std::vector<char> bytes;
+  // Per level: every GEMM's descriptor, split over k by its shape alone (plan_splits:
+  // the split a lone launch of that GEMM would use to fill the chip), so a request's
+  // bits never depend on the batch; split GEMMs write slabs that the level's reduction
+  // adds in slab order.
+  std::vector<std::vector<GemmDesc>> lev(levels);
+  std::vector<std::vector<ReduceDesc>> red(levels);
+  std::vector<size_t> slab_bytes(levels, 0);
   for (int L = 0; L < levels; ++L) {
-    std::vector<GemmDesc> d;
     for (int r = 0; r < count; ++r) {
       const fg_chain_req &q = reqs[r];
       if (q.num_gemm <= L) continue;
@@ -351,29 +345,76 @@ int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream) {
       char *x0 = spill ? spill + off[r] : nullptr, *x1 = x0 ? x0 + slot[r] : nullptr;
       const void *Bin = L == 0 ? q.B : ((L - 1) & 1 ? x1 : x0);
       void *Cout = last ? q.C : (L & 1 ? x1 : x0);
+      const int kind = kind_of(q.dtype, q.f32_mode), epi = last ? EPI_STORE : EPI_REQUANT;
       GemmDesc g;
-      gemm_desc(&g, kind_of(q.dtype, q.f32_mode), last ? EPI_STORE : EPI_REQUANT, q.A[L], Bin, Cout, q.M[L], q.N,
-                q.K[L]);
-      d.push_back(g);
+      gemm_desc(&g, kind, epi, q.A[L], Bin, Cout, q.M[L], q.N, q.K[L]);
+      const int sp = gemm_desc_split(&g, plan_splits(q.M[L], q.N, q.K[L], kind), nullptr);
+      if (sp > 1) {
+        // slab offset for now; made a pointer once the level's slabs are allocated
+        g.C = (void *)(uintptr_t)slab_bytes[L];
+        red[L].push_back({(const void *)(uintptr_t)slab_bytes[L], Cout, q.M[L] * q.N, sp, kind, epi, 0});
+        slab_bytes[L] += ((size_t)sp * q.M[L] * q.N * 4 + 255) & ~(size_t)255;
+      }
+      lev[L].push_back(g);
     }
+  }
+  const size_t slab_max = *std::max_element(slab_bytes.begin(), slab_bytes.end());
+  char *slabs = nullptr;
+  if (slab_max > 0 && (rc = scratch(s, "lvslab", slab_max, (void **)&slabs))) return rc;
+  // tables: a level's tiles with the longest k-slices first (kper over the kind's MFMA
+  // rate), then its reduction
+  auto tile_cost = [](const GemmDesc &d) {
+    const double rate = d.kind == K_I8 ? 32 : d.kind == K_BF16 ? 16 : d.kind == K_F32X3 ? 16.0 / 3 : 1;
+    return (double)d.kper / rate;
+  };
+  struct Part {
+    bool reduce;
+    int items;
+    size_t off;
+  };
+  std::vector<Part> parts;
+  std::vector<char> bytes;
+  auto put_table = [&](const void *hdr, size_t hsz, const void *descs, size_t dbytes, const std::vector<int2> &items,
+                       bool reduce) {
+    const size_t at = (bytes.size() + 255) & ~(size_t)255;
+    const size_t doff = align16(hsz), ioff = align16(doff + dbytes);
+    bytes.resize(at + ioff + items.size() * sizeof(int2), 0);
+    memcpy(bytes.data() + at, hdr, hsz);
+    memcpy(bytes.data() + at + doff, descs, dbytes);
+    memcpy(bytes.data() + at + ioff, items.data(), items.size() * sizeof(int2));
+    parts.push_back({reduce, (int)items.size(), at});
+  };
+  for (int L = 0; L < levels; ++L) {
+    std::vector<GemmDesc> &d = lev[L];
+    for (GemmDesc &g : d)
+      if (g.splits > 1) g.C = slabs + (uintptr_t)g.C;
     std::stable_sort(d.begin(), d.end(),
                      [&](const GemmDesc &a, const GemmDesc &b) { return tile_cost(a) > tile_cost(b); });
     std::vector<int2> items;
     for (size_t j = 0; j < d.size(); ++j) {
-      const long tiles = (d[j].M + LEVEL_TILE - 1) / LEVEL_TILE * d[j].tiles_n;
-      for (long t = 0; t < tiles; ++t) items.push_back(make_int2((int)j, (int)t));
+      const long units = (d[j].M + LEVEL_TILE - 1) / LEVEL_TILE * d[j].tiles_n * d[j].splits;
+      for (long t = 0; t < units; ++t) items.push_back(make_int2((int)j, (int)t));
     }
     LevelTable h{};
     h.num_descs = (int)d.size();
     h.num_items = (int)items.size();
     h.descs_off = (int)align16(sizeof(LevelTable));
     h.items_off = (int)align16(h.descs_off + d.size() * sizeof(GemmDesc));
-    const size_t at = (bytes.size() + 255) & ~(size_t)255;
-    bytes.resize(at + h.items_off + items.size() * sizeof(int2), 0);
-    memcpy(bytes.data() + at, &h, sizeof h);
-    memcpy(bytes.data() + at + h.descs_off, d.data(), d.size() * sizeof(GemmDesc));
-    memcpy(bytes.data() + at + h.items_off, items.data(), items.size() * sizeof(int2));
-    parts.push_back({h.num_items, at});
+    put_table(&h, sizeof h, d.data(), d.size() * sizeof(GemmDesc), items, false);
+    if (red[L].empty()) continue;
+    std::vector<int2> ritems;
+    for (size_t j = 0; j < red[L].size(); ++j) {
+      ReduceDesc &rd = red[L][j];
+      rd.P = slabs + (uintptr_t)rd.P;
+      const long chunks = (rd.MN + 255) / 256;
+      for (long c = 0; c < chunks; ++c) ritems.push_back(make_int2((int)j, (int)c));
+    }
+    ReduceTable rh{};
+    rh.num_descs = (int)red[L].size();
+    rh.num_items = (int)ritems.size();
+    rh.descs_off = (int)align16(sizeof(ReduceTable));
+    rh.items_off = (int)align16(rh.descs_off + red[L].size() * sizeof(ReduceDesc));
+    put_table(&rh, sizeof rh, red[L].data(), red[L].size() * sizeof(ReduceDesc), ritems, true);
   }
   StreamRing *R = ring_for(s);
   Slot &sl = R->slot[R->next];
@@ -382,9 +423,12 @@ int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream) {
   memcpy(sl.host, bytes.data(), bytes.size());
   if ((rc = hip_status(hipMemcpyAsync(sl.dev, sl.host, bytes.size(), hipMemcpyHostToDevice, s), "H2D level tables")))
     return rc;
-  for (const Part &p : parts)
-    if ((rc = hip_status(launch_grouped_gemm((const char *)sl.dev + p.off, p.items, s), "grouped level launch")))
+  for (const Part &p : parts) {
+    const char *t = (const char *)sl.dev + p.off;
+    if ((rc = hip_status(p.reduce ? launch_grouped_reduce(t, p.items, s) : launch_grouped_gemm(t, p.items, s),
+                         p.reduce ? "grouped level reduction" : "grouped level launch")))
       return rc;
+  }
   return hip_status(hipEventRecord(sl.done, s), "hipEventRecord");
 }
 

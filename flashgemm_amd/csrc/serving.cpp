@@ -367,8 +367,8 @@ int submit_at(fg_server *S, int set_id, long N, const void *B, void *C, long *re
   r.B = B;
   r.C = C;
   r.bytes = (size_t)w.K[0] * N * e + (size_t)w.M[w.num_gemm - 1] * N * 4;
-  // A request that cannot ride a grouped launch runs alone (without split-K) and takes
-  // library scratch (spill buffers, packed weights): that is charged too.
+  // A request that cannot ride the on-chip grouped launch runs in the level launch and
+  // takes library scratch (two intermediates, split-K slabs): that is charged too.
   fg_chain_req x;
   memset(&x, 0, sizeof x);
   x.dtype = w.dtype;
@@ -385,7 +385,7 @@ int submit_at(fg_server *S, int set_id, long N, const void *B, void *C, long *re
   const int fits = S->cfg.per_gemm_baseline ? 0 : fg_grouped_fits(&x);
   if (fits < 0) return fits;
   if (!fits)
-    r.bytes += chain_scratch_bound(w.dtype, w.f32_mode, N, w.num_gemm, w.M, w.K, true);
+    r.bytes += chain_scratch_bound(w.dtype, w.f32_mode, N, w.num_gemm, w.M, w.K, false);
   if (r.bytes + S->weight_bytes > S->cfg.hbm_budget_bytes)
     return set_error(FG_ERR_BUDGET, "request of %zu bytes can never fit the budget", r.bytes);
   r.arrival = arrival;

@@ -249,12 +249,14 @@ int fg_grouped_fits(const fg_chain_req *req);
 int fg_grouped_tile(const fg_chain_req *req);
 /* Every request unfused, level by level: GEMM i of all requests (i = 1 .. the longest
  * chain, any mix of dtypes and modes) runs in one launch, 128 x 128 output tiles of
- * all of them in one grid, each tile over its whole K; intermediates go to the
- * stream's scratch in the next operand's type (as fg_chain's unfused path writes
- * them). Each request's result has the bits of fg_chain with fuse = FG_FUSE_OFF and
- * splits = 1. For batches of requests too large for the on-chip grouped launch: one
- * launch per level instead of one per GEMM per request. Weights as given (a_packed
- * must be 0). Asynchronous. */
+ * all of them in one grid; a GEMM whose tiles alone would not fill the chip is split
+ * over k as fg_chain's planner splits a lone GEMM (the split depends on its shape
+ * only), its slabs added in slab order by one reduction launch per level.
+ * Intermediates go to the stream's scratch in the next operand's type (as fg_chain's
+ * unfused path writes them). A request's result has the same bits in any batch (and
+ * alone); INT8 results equal every other path's. For batches of requests too large for
+ * the on-chip grouped launch: a few launches per level instead of one or more per GEMM
+ * per request. Weights as given (a_packed must be 0). Asynchronous. */
 int fg_grouped_levels(int count, const fg_chain_req *reqs, void *stream);
 
 /* A captured hipGraph of [descriptor-table upload -> grouped kernel], replayed per

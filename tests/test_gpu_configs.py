@@ -289,10 +289,9 @@ def test_serving_trace_1000_requests(gpu):
     """1,000 requests of the serving workload (64 resident weight sets of 2-4 GEMMs,
     widths to 2048, request widths log-uniform in 1..4096, 50/30/20 % BF16/INT8/FP32)
     with Poisson arrivals at 20,000 requests/s through the engine: every result is bit
-    for bit the single-launch result of the same request (fg_chain alone with the
-    engine's plan for it: default options when it fits the on-chip grouped tile, else
-    unfused without split-K, the level launch's arithmetic), and 48 sampled requests
-    match the oracle on sampled columns."""
+    for bit the result of the same request launched alone with the engine's plan for
+    it (fg_chain when it fits the on-chip grouped tile, else the level launch), and 48
+    sampled requests match the oracle on sampled columns."""
     import torch
     sets = W.serving_sets(torch)
     set_of, Ns, gaps = W.serving_trace(1000, len(sets))
@@ -322,10 +321,11 @@ def test_serving_trace_1000_requests(gpu):
     for i, (s, n) in enumerate(zip(set_of, Ns)):
         st = sets[s]
         w = st["w"]
-        fits = fg.grouped_fits(fg.chain_req(st["dtype"], view(single, i), st["B"], st["As"], N=int(n), Ms=w[1:],
-                                            Ks=w[:-1]))
-        fg.chain(st["dtype"], view(single, i), st["B"], st["As"], N=int(n), Ms=w[1:], Ks=w[:-1],
-                 o=None if fits else fg.opts(fuse=fg.FUSE_OFF, splits=1))
+        req = fg.chain_req(st["dtype"], view(single, i), st["B"], st["As"], N=int(n), Ms=w[1:], Ks=w[:-1])
+        if fg.grouped_fits(req):
+            fg.chain(st["dtype"], view(single, i), st["B"], st["As"], N=int(n), Ms=w[1:], Ks=w[:-1])
+        else:
+            fg.grouped_levels([req])
     torch.cuda.synchronize()
     differ = [i for i in range(len(Ns)) if not torch.equal(view(arena, i), view(single, i))]
     assert not differ, f"{len(differ)} requests differ from their single launch, first {differ[:5]}"

@@ -1,8 +1,9 @@
 """GPU tests of the grouped level launch (fg_grouped_levels): many chains run unfused,
-GEMM i of all of them (any mix of dtypes) in one launch. Every request's result must
-have the bits of the same chain through fg_chain with fusion off and no split-K (each
-tile runs the whole K in the same order, whichever launch carries it), INT8 must equal
-the oracle, and FP32 / BF16 / split-bf16 stay within the parity bars."""
+GEMM i of all of them (any mix of dtypes) in one launch, GEMMs with few tiles split over
+k by their shape alone. Every request's result must have the same bits in a batch as
+launched alone (the split and the slab order do not depend on the batch); INT8 must
+equal the oracle and the unsplit unfused chain (integer sums are exact in any order),
+and FP32 / BF16 / split-bf16 stay within the parity bars."""
 import numpy as np
 import pytest
 
@@ -27,6 +28,15 @@ def make(rng, specs):
 
 
 def alone(q):
+    """The same request through the level launch on its own."""
+    import torch
+    C = torch.empty_like(q["C"])
+    fg.grouped_levels([fg.chain_req(q["dtype"], C, q["dB"], q["dA"], N=q["N"], f32_mode=q["mode"])])
+    torch.cuda.synchronize()
+    return C.cpu().numpy()
+
+
+def unfused(q):
     """The same chain through fg_chain, unfused, without split-K."""
     import torch
     C = torch.empty_like(q["C"])
@@ -83,6 +93,8 @@ def test_levels_mixed_batch_same_bits_as_alone(gpu):
               flush=True)
         a = alone(q)
         assert np.array_equal(g.view(np.uint32), a.view(np.uint32)), f"request {i}: {(g != a).sum()} differ"
+        if q["dtype"] == fg.DTYPE_INT8:
+            assert np.array_equal(g, unfused(q))
         check(q, g)
 
 
@@ -98,6 +110,8 @@ def test_levels_serving_sized_chains(gpu):
         got = run_levels(qs)
         for q, g in zip(qs, got):
             assert np.array_equal(g.view(np.uint32), alone(q).view(np.uint32))
+            if q["dtype"] == fg.DTYPE_INT8:
+                assert np.array_equal(g, unfused(q))
             check(q, g)
 
 

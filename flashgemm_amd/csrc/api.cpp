@@ -69,6 +69,7 @@ int scratch(hipStream_t s, const char *name, size_t bytes, void **out) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto &slot = g_scratch[s].bufs[name];
   if (slot.second < bytes) {
+    const size_t old = slot.second;
     if (slot.first) {
       // the old buffer may still be read by queued work on this stream
       hipError_t e = hipStreamSynchronize(s);
@@ -77,7 +78,12 @@ int scratch(hipStream_t s, const char *name, size_t bytes, void **out) {
       slot.first = nullptr;
       slot.second = 0;
     }
+    // grow geometrically (at least double, whole MiB past one): a growth synchronizes
+    // the stream, and workloads whose sizes creep up (a serving engine's batches)
+    // would otherwise stall on one growth after another
     size_t cap = bytes < 256 ? 256 : bytes;
+    if (cap < 2 * old) cap = 2 * old;
+    if (cap > (1u << 20)) cap = (cap + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
     hipError_t e = hipMalloc(&slot.first, cap);
     if (e != hipSuccess) return hip_status(e, "hipMalloc(scratch)");
     slot.second = cap;

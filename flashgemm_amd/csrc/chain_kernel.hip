@@ -607,11 +607,12 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   int cs[TNW];
 #pragma unroll
   for (int j = 0; j < TNW; ++j) cs[j] = 0;
-#pragma unroll
-  for (int i = 0; i < RP; ++i) {
+  // A block of 32 real rows (every block but a ragged last one) takes a path with no
+  // row masks: a wave-uniform branch, so the mask arithmetic is not computed (and
+  // selected away) for every value of every full block.
+  auto block = [&](const int i, auto full_c) {
+    constexpr bool full = decltype(full_c)::value;
     const int rbase = rb0 + i * 32;
-    if (rbase >= Mp) continue;
-    const bool full = rbase + 32 <= M;
     int rs = P.ybase[gi] + rbase;  // arena row of the block, a multiple of 32
     if (rs >= S) rs -= S;
     char *const blk = (char *)arena + (rs / T::KG) * TN * 16 + lane_off;
@@ -668,6 +669,13 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
         }
       }
     }
+  };
+#pragma unroll
+  for (int i = 0; i < RP; ++i) {
+    const int rbase = rb0 + i * 32;
+    if (rbase >= Mp) continue;
+    if (rbase + 32 <= M) block(i, std::true_type{});
+    else block(i, std::false_type{});
   }
   if (KIND == K_I8)
 #pragma unroll
@@ -755,9 +763,16 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       if (KIND == K_F32X3) dst[i].v[1] = *(const uint4 *)(p + 1024 + lo);
     }
   };
+  // Issued strictly in step order: the loop waits for step s of a chunk with a count
+  // of the loads issued after it, and a reordered burst (step 0 issued last) made the
+  // wait analysis assume so on every later chunk too, a near-full drain of the stream
+  // once per two chunks.
   auto load_chunk = [&](const char *base, unsigned lo, AFrag<KIND>(&dst)[PF][RP]) {
 #pragma unroll
-    for (int s = 0; s < PF; ++s) load_step(base, lo, s, dst[s]);
+    for (int s = 0; s < PF; ++s) {
+      load_step(base, lo, s, dst[s]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
   };
   constexpr size_t CHUNK = (size_t)PF * STEP;
 
@@ -773,18 +788,22 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     }
   };
 
-  if (KIND == K_I8) {
-    for (int t = fresh_tid(); t < 2 * TN; t += NTH) colsum[t] = 0;
-    __syncthreads();
-  }
   // invariant: at the start of each pass fa[0] holds (or has in flight) chunk 0
   const char *apk = first_ptr(0);
   unsigned alane = lane_of(0);  // lane offset of the GEMM apk belongs to
   const long K1c = (P.K[0] + CS - 1) / CS * CS;
   // A wave with no rows in GEMM 0 loads (and ignores) wave 0's first chunk, which
   // always exists: the loads stay unconditional and the wait counts after them exact.
-  stage_b<KIND, TN, IMG>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img,
-                         [&] { load_chunk(apk ? apk : (const char *)P.A[0], alane, fa[0]); });
+  // Between the tile's first loads and their LDS writes: the weight prefetch, and
+  // (INT8) the clearing of the column sums the tile's writes add into, with the
+  // barrier that orders the two (so it overlaps the tile's load latency).
+  stage_b<KIND, TN, IMG>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img, [&] {
+    load_chunk(apk ? apk : (const char *)P.A[0], alane, fa[0]);
+    if (KIND == K_I8) {
+      for (int t = fresh_tid(); t < 2 * TN; t += NTH) colsum[t] = 0;
+      __syncthreads();
+    }
+  });
   if (EPX) {
     // every GEMM's per-row bias and scale into LDS (bias bits, then scales, rows of
     // GEMM g after those of GEMMs < g): the epilogues read them with LDS loads, so
@@ -967,6 +986,16 @@ __device__ __forceinline__ int xcd_tile(int b, int G) {
 template <int KIND, int TNB, bool IMG, bool EPX, int G, bool RAWA = false>
 __global__ __launch_bounds__(NTH) void chain_kernel(ChainParams P) {
   extern __shared__ uint4 arena[];
+  {
+    // Touch every 64-byte line of the arguments with independent scalar loads: one
+    // round trip, instead of one serialized miss per line where a field is first used
+    // (about 0.5 us of the kernel's start).
+    const unsigned *kp = reinterpret_cast<const unsigned *>(&P);
+    unsigned kw = 0;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(ChainParams) / 4); i += 16) kw ^= kp[i];
+    asm volatile("" ::"s"(kw));
+  }
   chain_tile<KIND, TNB, IMG, EPX, G, RAWA>(P, (long)xcd_tile(blockIdx.x, gridDim.x) * (TNB * 32), arena);
 }
 

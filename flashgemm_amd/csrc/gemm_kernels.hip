@@ -1005,7 +1005,12 @@ __device__ __forceinline__ void sk_load(const float *__restrict__ A, const float
     }
 }
 
-template <int SK_PD>
+// FULL: every quadrant inside C (M, N multiples of 64), every slice a whole SK_QGROUPS
+// groups inside K, K % 4 == 0 and A 16-byte aligned: no guarded loads and no data-dependent
+// branch in the unrolled slice, so each group waits only for its own loads. With the
+// guarded path compiled in, the merged control flow made every group wait for all
+// loads in flight (vmcnt(0)), which drained the prefetch ring once per group.
+template <int SK_PD, bool FULL>
 __global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict__ A, const float *__restrict__ B,
                                                          float *__restrict__ C, long M, long N, long K, long kper,
                                                          int tiles_n, int partial) {
@@ -1014,11 +1019,11 @@ __global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict
   // quadrant (wave >> 1, wave & 1) of the workgroup's 64 x 64 tile
   const long m0 = (long)(blockIdx.x / tiles_n) * SK_TILE + 32 * (wave >> 1);
   const long n0 = (long)(blockIdx.x % tiles_n) * SK_TILE + 32 * (wave & 1);
-  if (m0 >= M || n0 >= N) return;  // the quadrant lies outside C (no barrier follows)
+  if (!FULL && (m0 >= M || n0 >= N)) return;  // the quadrant lies outside C (no barrier follows)
   const long kb = (long)blockIdx.y * kper;
-  const long ke = kb + kper < K ? kb + kper : K;
+  const long ke = FULL ? kb + kper : (kb + kper < K ? kb + kper : K);
   const long row = m0 + l32, col = n0 + l32;
-  const bool inside = m0 + 32 <= M && n0 + 32 <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0;
+  const bool inside = FULL || (m0 + 32 <= M && n0 + 32 <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0);
   f32x16 acc, acc2;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -1027,10 +1032,10 @@ __global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict
   }
   // the slice holds at most SK_QGROUPS groups (kper == SK_QGROUPS * SK_GROUP); groups
   // past the end of K (the last slice) are skipped with a uniform branch
-  const int ng = (int)((ke - kb + SK_GROUP - 1) / SK_GROUP);
+  const int ng = FULL ? SK_QGROUPS : (int)((ke - kb + SK_GROUP - 1) / SK_GROUP);
   auto load = [&](int g, SkFrag &f) {
     const long k = kb + (long)g * SK_GROUP;
-    sk_load(A, B, M, N, K, ke, row, col, k, inside && k + SK_GROUP <= ke, h, f);
+    sk_load(A, B, M, N, K, ke, row, col, k, FULL || (inside && k + SK_GROUP <= ke), h, f);
   };
   auto mma = [&](const SkFrag &f) {
 #pragma unroll
@@ -1044,25 +1049,31 @@ __global__ __launch_bounds__(256) void skinny_f32_kernel(const float *__restrict
   // ring of SK_PD + 1 fragment buffers, the whole slice unrolled at compile time so
   // every buffer index is a constant (registers, not scratch) and the compiler sees
   // the full load / MFMA schedule (a runtime-length loop measured 13.2 against 9.5 us)
+  // (the scheduling barriers keep each group's loads SK_PD groups ahead of its MFMAs:
+  // left alone, the scheduler reuses a few registers and issues every load just before
+  // the MFMA that reads it)
   SkFrag f[SK_PD + 1];
 #pragma unroll
   for (int g = 0; g < SK_PD; ++g)
     if (g < ng) load(g, f[g]);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int g = 0; g < SK_QGROUPS; ++g) {
     if (g >= ng) break;
     if (g + SK_PD < ng) load(g + SK_PD, f[(g + SK_PD) % (SK_PD + 1)]);
+    __builtin_amdgcn_sched_barrier(0);
     mma(f[g % (SK_PD + 1)]);
+    __builtin_amdgcn_sched_barrier(0);
   }
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] += acc2[r];
   // lane holds rows 8(r/4) + 4h + (r%4) of column l32 of the quadrant
   float *out = partial ? C + (size_t)blockIdx.y * M * N : C;
-  if (col >= N) return;
+  if (!FULL && col >= N) return;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const long m = m0 + 8 * (r >> 2) + 4 * h + (r & 3);
-    if (m < M) out[m * N + col] = acc[r];
+    if (FULL || m < M) out[m * N + col] = acc[r];
   }
 }
 
@@ -1114,6 +1125,11 @@ __device__ __forceinline__ void sk_tile_load(const float *__restrict__ A, const 
   }
 }
 
+// FULL: the tile inside C (M, N multiples of 64), every slice whole and a multiple of
+// 8 groups (each wave an even number of groups), K % 4 == 0, A 16-byte aligned: every
+// load is unconditional (the prefetch past the slice's end re-reads the wave's current
+// group), so each group waits only for its own loads instead of all in flight.
+template <bool FULL>
 __global__ __launch_bounds__(256) void skinny_tile_kernel(const float *__restrict__ A, const float *__restrict__ B,
                                                           float *__restrict__ C, long M, long N, long K, long kper,
                                                           int tiles_n, int partial) {
@@ -1122,8 +1138,8 @@ __global__ __launch_bounds__(256) void skinny_tile_kernel(const float *__restric
   const int h = lane >> 5, l32 = lane & 31;
   const long m0 = (long)(blockIdx.x / tiles_n) * SK_TILE, n0 = (long)(blockIdx.x % tiles_n) * SK_TILE;
   const long kb = (long)blockIdx.y * kper;
-  const long ke = kb + kper < K ? kb + kper : K;
-  const bool inside = m0 + SK_TILE <= M && n0 + SK_TILE <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0;
+  const long ke = FULL ? kb + kper : (kb + kper < K ? kb + kper : K);
+  const bool inside = FULL || (m0 + SK_TILE <= M && n0 + SK_TILE <= N && (K & 3) == 0 && ((uintptr_t)A & 15) == 0);
   f32x16 acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -1133,7 +1149,7 @@ __global__ __launch_bounds__(256) void skinny_tile_kernel(const float *__restric
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
   SkTileFrag f[2];
   long k = kb + (long)wave * SK_GROUP;
-  if (k < ke) sk_tile_load(A, B, M, N, K, ke, m0, n0, k, inside && k + SK_GROUP <= ke, h, l32, f[0]);
+  if (FULL || k < ke) sk_tile_load(A, B, M, N, K, ke, m0, n0, k, FULL || (inside && k + SK_GROUP <= ke), h, l32, f[0]);
   auto mma = [&](const SkTileFrag &g) {
 #pragma unroll
     for (int t = 0; t < 4; ++t)
@@ -1145,7 +1161,20 @@ __global__ __launch_bounds__(256) void skinny_tile_kernel(const float *__restric
           for (int i = 0; i < 2; ++i)
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(g.a[t][i][s], g.b[t][j][s], acc[i][j], 0, 0, 0);
   };
-  for (; k < ke; k += 8 * SK_GROUP) {  // two groups per iteration, buffers alternating
+  if (FULL) {
+    for (; k < ke; k += 8 * SK_GROUP) {  // two groups per iteration, buffers alternating
+      const long k1 = k + 4 * SK_GROUP, k2 = k + 8 * SK_GROUP;
+      sk_tile_load(A, B, M, N, K, ke, m0, n0, k1, true, h, l32, f[1]);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(f[0]);
+      __builtin_amdgcn_sched_barrier(0);
+      sk_tile_load(A, B, M, N, K, ke, m0, n0, k2 < ke ? k2 : k, true, h, l32, f[0]);
+      __builtin_amdgcn_sched_barrier(0);
+      mma(f[1]);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  for (; !FULL && k < ke; k += 8 * SK_GROUP) {  // two groups per iteration, buffers alternating
     const long k1 = k + 4 * SK_GROUP, k2 = k + 8 * SK_GROUP;
     if (k1 < ke) sk_tile_load(A, B, M, N, K, ke, m0, n0, k1, inside && k1 + SK_GROUP <= ke, h, l32, f[1]);
     __builtin_amdgcn_sched_barrier(0);
@@ -1206,10 +1235,18 @@ hipError_t launch_skinny_f32(const float *A, const float *B, float *C, long M, l
   // that shape, rocprofv3). Long slices (tall shapes such as 8192 x 64 x 8192, 4096 k
   // per slice): every wave on the whole tile, which feeds each loaded fragment to two
   // MFMAs instead of one (~100 against 140 us for that shape).
-  if (kper == SK_QGROUPS * SK_GROUP)
-    hipLaunchKernelGGL(skinny_f32_kernel<2>, grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
+  // every quadrant of every 64 x 64 tile inside C, every slice whole
+  const bool full = M % SK_TILE == 0 && N % SK_TILE == 0 && K % kper == 0 && K % 4 == 0 && ((uintptr_t)A & 15) == 0;
+  const bool full_tile = full && kper % (8 * SK_GROUP) == 0;
+  if (kper == SK_QGROUPS * SK_GROUP && full)
+    hipLaunchKernelGGL((skinny_f32_kernel<2, true>), grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
+  else if (kper == SK_QGROUPS * SK_GROUP)
+    hipLaunchKernelGGL((skinny_f32_kernel<2, false>), grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n,
+                       partial);
+  else if (full_tile)
+    hipLaunchKernelGGL(skinny_tile_kernel<true>, grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
   else
-    hipLaunchKernelGGL(skinny_tile_kernel, grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
+    hipLaunchKernelGGL(skinny_tile_kernel<false>, grid, dim3(256), 0, s, A, B, C, M, N, K, kper, tiles_n, partial);
   return hipGetLastError();
 }
 

@@ -951,7 +951,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] += acc2[i][j][r];
       }
-      if (final_pass && !last && P.hold) held_rb0 = rb0;
+      if (final_pass && !last && ((P.hold >> gi) & 1)) held_rb0 = rb0;
       else store_rows_impl<KIND, TN, TNW, EPX>(P, gi, rb0, n0, arena, acc, csum_next, cb0);
     }
     if ((long)wave * RP * 32 >= Mp && next_gemm) {  // no rows here: fetch the next GEMM's first chunk
@@ -960,7 +960,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       load_chunk(apk, alane, fa[0]);
     }
     if (last) break;  // C is written; nothing in LDS is read again
-    if (!P.hold) {
+    if (!((P.hold >> gi) & 1)) {
       // Y_gi does not meet X_gi: its padding rows are cleared now and one barrier
       // both retires X_gi and publishes Y_gi
       zero_pad(gi);
@@ -1030,30 +1030,36 @@ constexpr int wave_groups(int kind, int tile_n) {
 
 // Circular arena of S rows: X_0 (the B tile) at row 0, Y_i right after X_i (mod S)
 // and X_{i+1} = Y_i; every region spans its k-extent rounded up to a chunk and starts
-// on a chunk boundary (S is a multiple of a chunk). When max_i (Kc_i + Mc_i) rows fit
-// the LDS at this tile width, Y_i never meets X_i and every pass is stored as soon as
-// it is done (*hold = 0). Otherwise the arena shrinks to what the held final passes
-// need (*hold = 1): all rows of Y_i but each wave's final pass are written while X_i
-// is still read, so they must not meet it: Kc_i + (passes_i - 1) * PASS <= S.
+// on a chunk boundary (S is a multiple of a chunk). Per GEMM: when Kc_i + Mc_i rows
+// fit the LDS at this tile width, Y_i never meets X_i and every pass is stored as soon
+// as it is done; otherwise GEMM i is held (bit i of *hold): all rows of Y_i but each
+// wave's final pass are written while X_i is still read, so they must not meet it,
+// Kc_i + (passes_i - 1) * PASS <= S, and the final passes are written over X_i after a
+// barrier. Only the GEMMs that need it are held (the BF16 3-chain at 64 columns holds
+// its second GEMM, 512 + 1024 rows, not its first, 512 + 512).
 int plan_arena(int kind, const long *M, const long *K, int n, int tile_n, int *xbase, int *ybase, int *hold,
                size_t budget = LDS_LIMIT) {
   const long cs = CS_of(kind);
   auto rc = [cs](long v) { return (v + cs - 1) / cs * cs; };
   auto r32 = [](long v) { return (v + 31) / 32 * 32; };
   const long pass = (long)(NW / wave_groups(kind, tile_n)) * RP_of(kind) * 32;  // rows per pass
-  long S = rc(K[0]), wide = rc(K[0]);
+  auto fits = [&](long rows) { return tile_n > 0 && arena_bytes(kind, (int)rc(rows), tile_n) <= budget; };
+  long S = rc(K[0]);
+  *hold = 0;
   for (int i = 0; i < n; ++i) {
     const long kc = rc(K[i]);
     S = std::max(S, kc);
-    wide = std::max(wide, kc);
     if (i < n - 1) {
-      const long passes = (r32(M[i]) + pass - 1) / pass;
-      S = std::max(S, std::max(rc(M[i]), kc + (passes - 1) * pass));
-      wide = std::max(wide, kc + rc(M[i]));
+      const long wide = kc + rc(M[i]);
+      if (fits(wide)) {
+        S = std::max(S, wide);
+      } else {
+        const long passes = (r32(M[i]) + pass - 1) / pass;
+        S = std::max(S, std::max(rc(M[i]), kc + (passes - 1) * pass));
+        *hold |= 1 << i;
+      }
     }
   }
-  *hold = tile_n <= 0 || arena_bytes(kind, (int)wide, tile_n) > budget;
-  if (!*hold) S = wide;
   // A pass (PASS rows) can be shorter than a chunk (waves split over column groups),
   // so the held bound Kc_i + (passes_i - 1) * PASS need not be a whole number of
   // chunks; S must be, or a region starting on a chunk boundary could run past S

@@ -268,7 +268,7 @@ struct ChainParams {
   int ybase[CHAIN_MAXG];    // arena row where Y_i is written (i < n-1)
   int num_gemm;
   int S;                    // arena rows
-  int hold;                 // Y_i may overlap X_i: final passes are held (see plan_arena)
+  int hold;                 // bit i: Y_i may overlap X_i, GEMM i's final passes are held (plan_arena)
   const void *B;
   void *C;
   long N;

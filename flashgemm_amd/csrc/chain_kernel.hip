@@ -332,9 +332,21 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
       }
     }
   };
-  auto load = [&](int task, Raw(&x)[T::KG]) {
+  // INNER: the whole tile lies inside B (columns n0 .. n0 + TN - 1 < N, K1 a whole
+  // number of chunks, 32-bit offsets): a task's rows are one 32-bit base plus a uniform
+  // row stride, with no clamps and no edge zeroing (the generic path keeps 64-bit clamps
+  // and masks live, some hundred vector instructions per wave before its first load)
+  const unsigned rbu = (unsigned)N * T::ESZ, cbase = (unsigned)n0 * T::ESZ;
+  auto load = [&](auto inner_c, int task, Raw(&x)[T::KG]) {
     if constexpr (IMG) {
       load_img(task, x);
+      return;
+    }
+    if constexpr (decltype(inner_c)::value) {
+      const unsigned g = (unsigned)task / (TN / 4), cb = (unsigned)task % (TN / 4);
+      const unsigned o0 = g * T::KG * rbu + cbase + cb * 4 * T::ESZ;
+#pragma unroll
+      for (int r = 0; r < T::KG; ++r) x[r] = *(const Raw *)(B + (o0 + (unsigned)r * rbu));
       return;
     }
     const int g = task / (TN / 4), cb = task % (TN / 4);
@@ -365,11 +377,11 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
   auto comp = [](const uint4 &v, int c) { return c == 0 ? v.x : (c == 1 ? v.y : (c == 2 ? v.z : v.w)); };
   // Transpose a task (KG rows x 4 columns) into its 4 slots: byte permutes for INT8
   // (a 4 x 4 byte transpose is 8 v_perm_b32) and BF16 (one v_perm_b32 per pair of rows).
-  auto put = [&](int task, Raw(&x)[T::KG]) {
+  auto put = [&](auto inner_c, int task, Raw(&x)[T::KG]) {
     const int g = task / (TN / 4), cb = task % (TN / 4);
     const long k0 = (long)g * T::KG;
     const bool nok = n0 + cb * 4 < N;
-    if (!(nok && k0 + T::KG <= K1)) {  // edge task
+    if (!decltype(inner_c)::value && !(nok && k0 + T::KG <= K1)) {  // edge task
 #pragma unroll
       for (int r = 0; r < T::KG; ++r)
         if (!(nok && k0 + r < K1)) x[r] = Raw{};
@@ -423,19 +435,25 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
     }
   };
   const int tid = fresh_tid();
+  auto drive = [&](auto inner_c) {
 #pragma unroll
-  for (int f = 0; f < FIRST; ++f)
-    if (tid + f * NTH < tasks) load(tid + f * NTH, x[f]);
-  __builtin_amdgcn_sched_barrier(0);
-  issue();
-  __builtin_amdgcn_sched_barrier(0);
+    for (int f = 0; f < FIRST; ++f)
+      if (tid + f * NTH < tasks) load(inner_c, tid + f * NTH, x[f]);
+    __builtin_amdgcn_sched_barrier(0);
+    issue();
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int f = 0; f < FIRST; ++f)
-    if (tid + f * NTH < tasks) put(tid + f * NTH, x[f]);
-  for (int task = tid + FIRST * NTH; task < tasks; task += NTH) {
-    load(task, x[0]);
-    put(task, x[0]);
-  }
+    for (int f = 0; f < FIRST; ++f)
+      if (tid + f * NTH < tasks) put(inner_c, tid + f * NTH, x[f]);
+    for (int task = tid + FIRST * NTH; task < tasks; task += NTH) {
+      load(inner_c, task, x[0]);
+      put(inner_c, task, x[0]);
+    }
+  };
+  // wave-uniform (n0, N and K1 are the workgroup's), so every thread takes one branch
+  // and the barrier inside issue() is reached by all of them
+  if (!IMG && off32 && n0 + TN <= N && K1p == K1) drive(std::true_type{});
+  else drive(std::false_type{});
 }
 
 // Intermediate epilogue in the reference's layout (ChainParams::refl). The reference's

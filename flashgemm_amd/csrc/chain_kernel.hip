@@ -219,7 +219,7 @@ __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KI
 // there, so the tile's loads are not queued behind it. INT8 adds each column's sum of
 // s8 X_0 into colsum (zeroed beforehand). IMG: B is the patch matrix of the image at
 // B (geometry img), gathered element by element; N % 4 and alignment do not matter.
-template <int KIND, int TN, bool IMG, typename F>
+template <int KIND, int TN, bool IMG, bool FASTB, typename F>
 __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B, long K1, long K1p, long N, long n0,
                                         int *colsum, const ImgGeom &img, F &&issue) {
   using T = CT<KIND>;
@@ -451,9 +451,17 @@ __device__ __forceinline__ void stage_b(uint4 *arena, int lo_off, const char *B,
     }
   };
   // wave-uniform (n0, N and K1 are the workgroup's), so every thread takes one branch
-  // and the barrier inside issue() is reached by all of them
-  if (!IMG && off32 && n0 + TN <= N && K1p == K1) drive(std::true_type{});
-  else drive(std::false_type{});
+  // and the barrier inside issue() is reached by all of them. The grouped kernel
+  // (FASTB false) keeps the generic path only: its code, four kinds of chain tile,
+  // already exceeds the instruction cache, and the second copy of the staging code
+  // cost its mixed batches more than the fast path saved.
+  if constexpr (FASTB) {
+    if (!IMG && off32 && n0 + TN <= N && K1p == K1) {
+      drive(std::true_type{});
+      return;
+    }
+  }
+  drive(std::false_type{});
 }
 
 // Intermediate epilogue in the reference's layout (ChainParams::refl). The reference's
@@ -722,7 +730,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
 // NW / G waves of its group. Wide tiles with G = TNB (one 32-column block per wave)
 // keep every wave busy in chains whose M_i are a few row blocks, where G = 1 would
 // leave most waves without rows.
-template <int KIND, int TNB, bool IMG = false, bool EPX = false, int G = 1, bool RAWA = false>
+template <int KIND, int TNB, bool IMG = false, bool EPX = false, int G = 1, bool RAWA = false, bool FASTB = true>
 __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, uint4 *arena) {
   using T = CT<KIND>;
   using Acc = typename T::Acc;
@@ -828,7 +836,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   // Between the tile's first loads and their LDS writes: the weight prefetch, and
   // (INT8) the clearing of the column sums the tile's writes add into, with the
   // barrier that orders the two (so it overlaps the tile's load latency).
-  stage_b<KIND, TN, IMG>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img, [&] {
+  stage_b<KIND, TN, IMG, FASTB>(arena, lo_off, (const char *)P.B, P.K[0], K1c, P.N, n0, colsum, P.img, [&] {
     load_chunk(apk ? apk : (const char *)P.A[0], alane, fa[0]);
     if (KIND == K_I8) {
       for (int t = fresh_tid(); t < 2 * TN; t += NTH) colsum[t] = 0;
@@ -1158,17 +1166,17 @@ __global__ __launch_bounds__(NTH) void grouped_chain_kernel(const char *__restri
     const long n0 = (long)w.y * P.gtnb * 32;
     if (WIDE && P.gtnb == 2) {
       switch (P.kind) {
-        case K_F32: chain_tile<K_F32, 2>(P, n0, arena); break;
-        case K_BF16: chain_tile<K_BF16, 2>(P, n0, arena); break;
-        case K_I8: chain_tile<K_I8, 2>(P, n0, arena); break;
-        case K_F32X3: chain_tile<K_F32X3, 2>(P, n0, arena); break;
+        case K_F32: chain_tile<K_F32, 2, false, false, 1, false, false>(P, n0, arena); break;
+        case K_BF16: chain_tile<K_BF16, 2, false, false, 1, false, false>(P, n0, arena); break;
+        case K_I8: chain_tile<K_I8, 2, false, false, 1, false, false>(P, n0, arena); break;
+        case K_F32X3: chain_tile<K_F32X3, 2, false, false, 1, false, false>(P, n0, arena); break;
       }
     } else {
       switch (P.kind) {
-        case K_F32: chain_tile<K_F32, GT_F32>(P, n0, arena); break;
-        case K_BF16: chain_tile<K_BF16, GT_BF16>(P, n0, arena); break;
-        case K_I8: chain_tile<K_I8, GT_I8>(P, n0, arena); break;
-        case K_F32X3: chain_tile<K_F32X3, GT_F32X3>(P, n0, arena); break;
+        case K_F32: chain_tile<K_F32, GT_F32, false, false, 1, false, false>(P, n0, arena); break;
+        case K_BF16: chain_tile<K_BF16, GT_BF16, false, false, 1, false, false>(P, n0, arena); break;
+        case K_I8: chain_tile<K_I8, GT_I8, false, false, 1, false, false>(P, n0, arena); break;
+        case K_F32X3: chain_tile<K_F32X3, GT_F32X3, false, false, 1, false, false>(P, n0, arena); break;
       }
     }
     __syncthreads();  // the next tile restages the arena

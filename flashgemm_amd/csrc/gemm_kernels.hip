@@ -75,6 +75,10 @@ __device__ __forceinline__ int slot(int g, int r) {
 }
 
 // ---------------------------------------------------------------- staging --
+// Word 3 of a raw buffer resource on gfx9 (32-bit data format, no swizzle, no index
+// stride): what the interior tiles' buffer loads use.
+constexpr int BUF_DW3 = 0x00020000;
+
 template <int KIND, class TL>
 struct Stager {
   using T = KT<KIND>;
@@ -95,12 +99,19 @@ struct Stager {
   // 4 per thread (INT8: the u8 -> s8 bias of A is applied here).
   __device__ __forceinline__ void load_rows(const char *A, long M, long K, long kend, long m0, long k0,
                                             bool vec, int t, uint32_t (&a)[16]) {
-    if (vec && m0 + BT <= M && k0 + T::BK <= kend) {  // interior tile (uniform branch)
+    if (vec && m0 + BT <= M && k0 + T::BK <= kend && (size_t)BT * K * T::ESZ < (1u << 31)) {
+      // interior tile (uniform branch), as buffer loads: the tile's corner is the
+      // resource base, the thread's row and 16-byte unit one vector offset and the
+      // q-th row step a scalar one, so no per-load 64-bit address arithmetic (unit
+      // u = t + NT q lies in row (t >> 3) + q NT / 8, same column unit t & 7)
+      const char *base = A + ((size_t)m0 * K + k0) * T::ESZ;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, BUF_DW3);
+      const unsigned rowb = (unsigned)K * T::ESZ;
+      const unsigned vo = (unsigned)(t >> 3) * rowb + (unsigned)(t & 7) * 16u;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        int u = t + NT * q;
-        const uint4 val = *(const uint4 *)(A + ((size_t)(m0 + (u >> 3)) * K + k0 + (long)(u & 7) * (16 / T::ESZ)) *
-                                                   T::ESZ);
+        const uint4 val = __builtin_bit_cast(
+            uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo, (int)((unsigned)(q * (NT / 8)) * rowb), 0));
         const uint32_t x = KIND == K_I8 ? 0x80808080u : 0u;  // u8 -> s8 bias
         a[4 * q + 0] = val.x ^ x; a[4 * q + 1] = val.y ^ x; a[4 * q + 2] = val.z ^ x; a[4 * q + 3] = val.w ^ x;
       }
@@ -173,18 +184,25 @@ struct Stager {
     long n = n0 + (long)cb * T::CW;
     long kb = k0 + (long)kg * T::RB;
     constexpr int RBYTES = T::CW * T::ESZ;  // bytes per row piece: 16, 8, 4 or 8
-    if (vec && n0 + BT <= N && k0 + T::BK <= kend) {  // interior tile (uniform branch)
+    if (vec && n0 + BT <= N && k0 + T::BK <= kend && (size_t)T::BK * N * T::ESZ < (1u << 31)) {
+      // interior tile (uniform branch), as buffer loads: the tile's corner is the
+      // resource base, the thread's first row and column block one vector offset, the
+      // r-th row a scalar one
+      const char *base = B + ((size_t)k0 * N + n0) * T::ESZ;
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, BUF_DW3);
+      const unsigned rowb = (unsigned)N * T::ESZ;
+      const unsigned vo = (unsigned)(kg * T::RB) * rowb + (unsigned)(cb * T::CW * T::ESZ);
 #pragma unroll
       for (int r = 0; r < T::RB; ++r) {
-        const char *p = B + ((size_t)(kb + r) * N + n) * T::ESZ;
+        const int so = (int)((unsigned)r * rowb);
         if (RBYTES == 16) {
-          uint4 v = *(const uint4 *)p;
+          const uint4 v = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)vo, so, 0));
           b[4 * r + 0] = v.x; b[4 * r + 1] = v.y; b[4 * r + 2] = v.z; b[4 * r + 3] = v.w;
         } else if (RBYTES == 8) {
-          uint2 v = *(const uint2 *)p;
+          const uint2 v = __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)vo, so, 0));
           b[2 * r + 0] = v.x; b[2 * r + 1] = v.y;
         } else {
-          b[r] = *(const uint32_t *)p;
+          b[r] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)vo, so, 0);
         }
       }
       return;

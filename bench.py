@@ -88,6 +88,24 @@ class Timer:
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps  # ms per call
 
+    def steady(self, fn, steps: int, warmup: int, reps: int = 3) -> float:
+        """The per-config timing: warm-up calls, then `reps` batches of `steps` timed
+        calls enqueued right behind them (no host synchronization in between, so the GPU
+        does not idle and drop its clock before the timed calls; each batch's events
+        bracket exactly its calls on the stream); the median batch, in ms per call."""
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        ev[0].record(self.stream)
+        for r in range(reps):
+            for _ in range(steps):
+                fn()
+            ev[r + 1].record(self.stream)
+        torch.cuda.synchronize()
+        per = sorted(ev[r].elapsed_time(ev[r + 1]) / steps for r in range(reps))
+        return per[reps // 2]
+
 
 def bench_main(torch, fg, timer, stream, steps, warmup):
     B, (A1, A2) = W.chain2_f32(torch)
@@ -128,10 +146,10 @@ def bench_main(torch, fg, timer, stream, steps, warmup):
                 inputs=(B, A1, A2))
 
 
-def bench_sgemm(torch, fg, timer, stream, M, N, K, mode, steps=20, warmup=3):
+def bench_sgemm(torch, fg, timer, stream, M, N, K, mode, steps=100, warmup=30):
     A, B = W.sgemm_operands(torch, M, N, K)
     C = torch.empty((M, N), device="cuda", dtype=torch.float32)
-    ms = timer.run(lambda: fg.sgemm(C, A, B, M, N, K, f32_mode=mode, stream=stream), steps, warmup)
+    ms = timer.steady(lambda: fg.sgemm(C, A, B, M, N, K, f32_mode=mode, stream=stream), steps, warmup)
     ref = A.double() @ B.double()
     err = float(torch.linalg.norm(C.double() - ref) / torch.linalg.norm(ref))
     flop = 2.0 * M * N * K
@@ -140,7 +158,7 @@ def bench_sgemm(torch, fg, timer, stream, M, N, K, mode, steps=20, warmup=3):
                 gbs=byts / ms / 1e6, frac_hbm=byts / ms / 1e6 / HBM_GBS, rel_err_vs_fp64=err)
 
 
-def bench_sgemm_t(torch, fg, timer, stream, M=2048, N=2048, K=2048, steps=20, warmup=3):
+def bench_sgemm_t(torch, fg, timer, stream, M=2048, N=2048, K=2048, steps=100, warmup=30):
     """FP32 GEMM with transposed operand storage (fg_sgemm_t: NT, TN, TT) beside NN;
     the reference's dispatcher computes NN only (singlegemm/FP32_ID23.c:39-43)."""
     gen = torch.Generator(device="cuda").manual_seed(5)
@@ -152,12 +170,12 @@ def bench_sgemm_t(torch, fg, timer, stream, M=2048, N=2048, K=2048, steps=20, wa
     out = {}
     for ta, tb in ((0, 0), (0, 1), (1, 0), (1, 1)):
         a, b = (At if ta else A), (Bt if tb else B)
-        ms = timer.run(lambda: fg.sgemm_t(ta, tb, C, a, b, M, N, K, stream=stream), steps, warmup)
+        ms = timer.steady(lambda: fg.sgemm_t(ta, tb, C, a, b, M, N, K, stream=stream), steps, warmup)
         out["NT"[ta] + "NT"[tb]] = dict(ms=ms, tflops=flop / ms / 1e9, frac_mfma=flop / ms / 1e9 / PEAK["fp32"])
     return out
 
 
-def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
+def bench_chain3(torch, fg, timer, stream, dtype, steps=100, warmup=30):
     """X(16384x512).W1(512x512).W2(512x1024).W3(1024x512) in the reference
     orientation: B = X^T (512 x 16384), A_i = W_i^T."""
     B, As = W.chain3(torch, dtype)
@@ -167,7 +185,7 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
     C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.int32 if dtype == fg.DTYPE_INT8 else torch.float32)
     gen = torch.Generator(device="cuda").manual_seed(33)  # epilogue-extension operands
     fused, tn = fg.chain_plan(dtype, N, Ms, Ks)
-    ms = timer.run(lambda: fg.chain(dtype, C, B, As, stream=stream), steps, warmup)
+    ms = timer.steady(lambda: fg.chain(dtype, C, B, As, stream=stream), steps, warmup)
     ms_kernel = None
     if fused:  # the chain kernel alone: weights packed once (resident, as in serving)
         packed = []
@@ -176,8 +194,8 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
             fg.pack_weight(dtype, a, a.shape[0], a.shape[1], pk, stream=stream)
             packed.append(pk)
         Ck = torch.empty_like(C)
-        ms_kernel = timer.run(lambda: fg.chain(dtype, Ck, B, packed, N=N, Ms=Ms, Ks=Ks, o=fg.opts(a_packed=1),
-                                               stream=stream), steps, warmup)
+        ms_kernel = timer.steady(lambda: fg.chain(dtype, Ck, B, packed, N=N, Ms=Ms, Ks=Ks, o=fg.opts(a_packed=1),
+                                                  stream=stream), steps, warmup)
         torch.cuda.synchronize()
         if not torch.equal(Ck, C):
             raise RuntimeError("pre-packed chain differs from the drop-in call")
@@ -192,10 +210,10 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
         scale = [torch.ones((m,), device="cuda") for m in Ms]
         epi = fg.epilogue(bias=bias, scale=scale, relu=[1] * len(Ms))
     Ce = torch.empty_like(C)
-    ms_epi = timer.run(lambda: fg.chain_epi(dtype, Ce, B, As, epi, stream=stream), steps, warmup)
+    ms_epi = timer.steady(lambda: fg.chain_epi(dtype, Ce, B, As, epi, stream=stream), steps, warmup)
     Cu = torch.empty_like(C)
-    ms_unfused = timer.run(lambda: fg.chain(dtype, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF), stream=stream),
-                           steps, warmup)
+    ms_unfused = timer.steady(lambda: fg.chain(dtype, Cu, B, As, o=fg.opts(fuse=fg.FUSE_OFF), stream=stream),
+                              steps, warmup)
     torch.cuda.synchronize()
     same = bool(torch.equal(C, Cu)) if dtype == fg.DTYPE_INT8 else \
         float(torch.linalg.norm((C - Cu).double()) / torch.linalg.norm(Cu.double()))
@@ -211,7 +229,7 @@ def bench_chain3(torch, fg, timer, stream, dtype, steps=20, warmup=3):
     return out
 
 
-def bench_conv(torch, fg, timer, stream, dtype, channels, size, ksize, Ms, steps=20, warmup=3):
+def bench_conv(torch, fg, timer, stream, dtype, channels, size, ksize, Ms, steps=100, warmup=30):
     """Convolution-shaped chain of the reference's conv tests (multigemm/test/test_int8_3.c:13-48,
     test_f32_3.c:14-48): GEMM 1 is a ksize x ksize convolution (pad ksize // 2) of a
     channels x size x size image, K1 = channels * ksize^2, N = size^2; then 1x1 layers.
@@ -232,13 +250,13 @@ def bench_conv(torch, fg, timer, stream, dtype, channels, size, ksize, Ms, steps
         C = torch.empty((Ms[-1], N), device="cuda", dtype=torch.float32)
         e = 2
     B = torch.empty((K1, N), device="cuda", dtype=X.dtype)
-    ms = timer.run(lambda: fg.chain_conv(dtype, C, X, g, As, stream=stream), steps, warmup)
+    ms = timer.steady(lambda: fg.chain_conv(dtype, C, X, g, As, stream=stream), steps, warmup)
     Cf = torch.empty_like(C)
 
     def formed():
         fg.im2col(dtype, B, X, g, stream=stream)
         fg.chain(dtype, Cf, B, As, stream=stream)
-    ms_formed = timer.run(formed, steps, warmup)
+    ms_formed = timer.steady(formed, steps, warmup)
     torch.cuda.synchronize()
     flop = 2.0 * N * sum(m * k for m, k in zip(Ms, Ks))
     peak = PEAK["int8" if dtype == fg.DTYPE_INT8 else "bf16"]
@@ -352,7 +370,7 @@ def verify_serving(torch, fg, sets, set_of, Ns, sample, kept, refs):
                 rel_err_vs_fp64=rel)
 
 
-def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
+def bench_precision(torch, fg, timer, stream, steps=10, warmup=3):
     B, As = W.precision_chain(torch)
     N, widths = B.shape[1], list(W.PRECISION_WIDTHS)
     Ms, Ks = widths[1:], widths[:-1]
@@ -364,7 +382,7 @@ def bench_precision(torch, fg, timer, stream, steps=3, warmup=1):
     for name, mode in (("fp32_mfma", fg.F32_MFMA), ("bf16x3", fg.F32_BF16X3), ("bf16x6", fg.F32_BF16X6)):
         C = torch.empty((widths[-1], N), device="cuda", dtype=torch.float32)
         o = fg.opts(f32_mode=mode)
-        ms = timer.run(lambda: fg.chain(fg.DTYPE_F32, C, B, As, o=o, stream=stream), steps, warmup)
+        ms = timer.steady(lambda: fg.chain(fg.DTYPE_F32, C, B, As, o=o, stream=stream), steps, warmup)
         d2, r2 = fg.error_norms(0, C, ref, C.numel(), stream=stream)
         out[name] = dict(ms=ms, tflops=flop / ms / 1e9, rel_err_vs_fp64=(d2 / r2) ** 0.5)
     e32 = max(out["fp32_mfma"]["rel_err_vs_fp64"], 1e-30)

@@ -387,7 +387,10 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
   using Acc = typename T::Acc;
   constexpr int BT = TL::BT, IM = TL::IM, JN = TL::JN, TILE_U4 = TL::TU4;
   static_assert(KG == 1 || ((KIND == K_F32 || KIND == K_F32X3) && EPI == EPI_STORE), "k groups: FP32 kinds, C");
-  const int grp = KG == 1 ? 0 : (int)(threadIdx.x / TL::NT);
+  // the group index is wave-uniform (NT is a multiple of 64); readfirstlane tells the
+  // compiler so, which keeps the staging's buffer resources in scalar registers (derived
+  // from a per-thread value they were rebuilt per lane in a loop around every load)
+  const int grp = KG == 1 ? 0 : __builtin_amdgcn_readfirstlane((int)(threadIdx.x / TL::NT));
   const int t = KG == 1 ? (int)threadIdx.x : (int)(threadIdx.x - grp * TL::NT), lane = t & 63, wave = t >> 6;
   const int wm = wave / TL::WN, wn = wave % TL::WN;
   uint4 *const smem = smem_all + grp * 4 * TILE_U4;

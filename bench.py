@@ -392,6 +392,61 @@ def bench_precision(torch, fg, timer, stream, steps=10, warmup=3):
     return out
 
 
+def bench_vendor(torch, timer, stream, ours):
+    """The vendor library on the same workloads: torch.mm (hipBLASLt / rocBLAS, TF32 off),
+    one call per GEMM, timed like the configs above, beside this build's time for the
+    same call (`ours`: config name -> ms). The vendor's chains round each intermediate
+    as this build does for FP32 and BF16 (BF16: its final output is bf16, not fp32, so
+    it writes half the C bytes); its INT8 chain is torch._int_mm, s8 x s8 (it has no
+    u8 x s8 product), with a saturating clamp between GEMMs: same shapes, a proxy for
+    the work, not the reference's arithmetic."""
+    torch.backends.cuda.matmul.allow_tf32 = False
+    out = {}
+
+    def add(name, flop, fn, our_key, steps=50, warmup=10):
+        try:
+            ms = timer.steady(fn, steps, warmup)
+        except Exception as e:  # e.g. no int8 GEMM in this torch build
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            return
+        r = dict(ms=ms, tflops=flop / ms / 1e9)
+        if ours.get(our_key):
+            r.update(ours_ms=ours[our_key], ours_speedup=ms / ours[our_key])
+        out[name] = r
+
+    for M, N, K, key in ((2048, 2048, 2048, "sgemm_2048_fp32_mfma"), (8192, 64, 8192, "sgemm_8192x64x8192"),
+                         (64, 64, 65536, "sgemm_64x64x65536")):
+        A, B = W.sgemm_operands(torch, M, N, K)
+        add(f"sgemm_{M}x{N}x{K}", 2.0 * M * N * K, lambda A=A, B=B: torch.mm(A, B), key)
+    B, (A1, A2) = W.chain2_f32(torch)
+    add("chain2_fp32_8192", 2.0 * 8192 * 2 * 256 * 256, lambda: torch.mm(A2, torch.mm(A1, B)), "main")
+    flop3 = 2.0 * 16384 * (512 * 512 + 1024 * 512 + 512 * 1024)
+    B, As = W.chain3(torch, W.DTYPE_BF16)
+    add("chain3_bf16_16384", flop3, lambda: torch.mm(As[2], torch.mm(As[1], torch.mm(As[0], B))),
+        "chain3_bf16_16384")
+    B, As = W.chain3(torch, W.DTYPE_INT8)
+    Bt = B.t().contiguous()  # _int_mm takes (m x k) . (k x n): run the transposed chain X . W1^T ...
+    Wt = [a.t().contiguous() for a in As]
+
+    def int8_chain():
+        y = Bt
+        for i, w in enumerate(Wt):
+            z = torch._int_mm(y, w)
+            y = z.clamp_(-128, 127).to(torch.int8) if i + 1 < len(Wt) else z
+        return y
+    add("chain3_int8_16384_s8s8_proxy", flop3, int8_chain, "chain3_int8_16384")
+    B, As = W.precision_chain(torch)
+
+    def f32_chain4():
+        y = B
+        for a in As:
+            y = torch.mm(a, y)
+        return y
+    flop4 = 2.0 * 16384 * sum(m * k for m, k in zip(W.PRECISION_WIDTHS[1:], W.PRECISION_WIDTHS[:-1]))
+    add("precision_fp32_4chain", flop4, f32_chain4, "precision_fp32_4chain", steps=10, warmup=3)
+    return out
+
+
 def stream_calibration(fg, stream, clock_ghz=2.4):
     """Per-CU rate of the fused chains' weight stream: every workgroup (one per CU)
     streams 4 MB through 8 waves x 16 loads of 1 KB in flight from a 256 KB region all
@@ -562,8 +617,9 @@ def main() -> None:
         ("precision_fp32_4chain", lambda: bench_precision(torch, fg, timer, stream)),
         ("serving_10k", lambda: bench_serving(torch, fg)),
     )
+    fns_only = dict(extra, vendor_torch_mm=lambda: bench_vendor(torch, timer, stream, {}))
     if args.only:
-        fns = dict(extra)
+        fns = fns_only
         if args.only not in fns:
             sys.exit(f"unknown config {args.only}; one of {sorted(fns)}")
         print(json.dumps({args.only: fns[args.only]()}))
@@ -583,6 +639,16 @@ def main() -> None:
                 configs[name] = fn()
             except Exception as e:  # report, keep the main line
                 configs[name] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.main_only:
+        log("vendor library")
+        ours = {k: v.get("ms") for k, v in configs.items() if isinstance(v, dict)}
+        ours["main"] = ms
+        prec = configs.get("precision_fp32_4chain", {})
+        ours["precision_fp32_4chain"] = prec.get("fp32_mfma", {}).get("ms")
+        try:
+            configs["vendor_torch_mm"] = bench_vendor(torch, timer, stream, ours)
+        except Exception as e:
+            configs["vendor_torch_mm"] = {"error": f"{type(e).__name__}: {e}"}
     cpu = None
     if not args.main_only:
         log("cpu baseline")

@@ -128,6 +128,7 @@ def lib() -> ctypes.CDLL:
         L.fg_to_f64.argtypes = [i, vp, vp, l, vp]
         L.fg_error_norms.argtypes = [i, vp, vp, l, ctypes.POINTER(ctypes.c_double), vp]
         L.fg_measure_stream.argtypes = [l, i, i, ctypes.POINTER(ctypes.c_float), vp]
+        L.fg_inject_alloc_failures.argtypes = [i]
         L.fg_init.argtypes = [i]
         L.fg_synchronize.argtypes = [vp]
         L.fg_last_error.restype = ctypes.c_char_p
@@ -321,6 +322,12 @@ def error_norms(x_kind: int, x, ref, n: int, stream=None):
     out = (ctypes.c_double * 2)()
     _check(lib().fg_error_norms(x_kind, _addr(x), _addr(ref), n, out, _stream(stream)))
     return out[0], out[1]
+
+
+def inject_alloc_failures(count: int) -> None:
+    """Make the library's next `count` device allocations fail as out of memory (tests of
+    the error paths: the call needing one raises FlashgemmError with ERR_NOMEM); 0 clears."""
+    _check(lib().fg_inject_alloc_failures(count))
 
 
 def measure_stream(slice_kb: int, rounds: int, workgroups: int, stream=None) -> float:

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -56,6 +57,20 @@ int ensure_init() {
 
 int num_cus() { return g_num_cu; }
 
+// ------------------------------------------------------- device allocation --
+// Every device allocation the library makes goes through dev_alloc, so a test can
+// make the next n of them fail as if the device were out of memory
+// (fg_inject_alloc_failures) and check that each caller reports FG_ERR_NOMEM and
+// leaves its state consistent.
+static std::atomic<int> g_inject_alloc{0};
+
+hipError_t dev_alloc(void **p, size_t bytes) {
+  int n = g_inject_alloc.load();
+  while (n > 0)
+    if (g_inject_alloc.compare_exchange_weak(n, n - 1)) return hipErrorOutOfMemory;
+  return hipMalloc(p, bytes);
+}
+
 // ------------------------------------------------------- scratch per stream --
 // Each stream owns named device buffers that only grow. Work on one stream is
 // ordered, so reusing its buffers across calls is safe; different streams never
@@ -84,8 +99,11 @@ int scratch(hipStream_t s, const char *name, size_t bytes, void **out) {
     size_t cap = bytes < 256 ? 256 : bytes;
     if (cap < 2 * old) cap = 2 * old;
     if (cap > (1u << 20)) cap = (cap + (1u << 20) - 1) & ~(size_t)((1u << 20) - 1);
-    hipError_t e = hipMalloc(&slot.first, cap);
-    if (e != hipSuccess) return hip_status(e, "hipMalloc(scratch)");
+    hipError_t e = dev_alloc(&slot.first, cap);
+    if (e != hipSuccess) {
+      slot.first = nullptr;
+      return hip_status(e, "hipMalloc(scratch)");
+    }
     slot.second = cap;
   }
   *out = slot.first;
@@ -1032,6 +1050,11 @@ int fg_error_norms(int x_kind, const void *x, const double *ref, long n, double 
   if ((rc = hip_status(hipMemcpyAsync(out, ws, 2 * sizeof(double), hipMemcpyDeviceToHost, s), "D2H norms")))
     return rc;
   return hip_status(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+int fg_inject_alloc_failures(int count) {
+  fg::g_inject_alloc.store(count > 0 ? count : 0);
+  return FG_OK;
 }
 
 int fg_measure_stream(long slice_kb, int rounds, int workgroups, float *ms, void *stream) {

@@ -164,7 +164,7 @@ int fg_autotune_chain(int dtype, int f32_mode, long N, int num_gemm, const long 
   // operands: seeded random data of the chain's dtype (timing only)
   std::vector<void *> bufs;
   auto alloc = [&](size_t elems, size_t bytes_per, void **p, unsigned long long seed) -> int {
-    hipError_t er = hipMalloc(p, elems * bytes_per);
+    hipError_t er = dev_alloc(p, elems * bytes_per);
     if (er != hipSuccess) return hip_status(er, "hipMalloc(autotune)");
     bufs.push_back(*p);
     return seed ? hip_status(launch_fill_random(esz, *p, (long)elems, seed, s), "fill") : FG_OK;

@@ -36,6 +36,9 @@ int hip_status(hipError_t e, const char *what);
 int ensure_init();
 int num_cus();
 // Named per-stream scratch buffer of at least `bytes` (grow-only).
+// Device allocation through the library's allocator: the next n calls fail as if the
+// device were out of memory after fg_inject_alloc_failures(n).
+hipError_t dev_alloc(void **p, size_t bytes);
 int scratch(hipStream_t s, const char *name, size_t bytes, void **out);
 // Frees the scratch buffers and the grouped-table ring of stream s (s synchronized).
 void release_stream_scratch(hipStream_t s);

@@ -204,7 +204,10 @@ int slot_reserve(Slot &s, size_t bytes) {
   s.cap = 0;
   size_t cap = std::max<size_t>(bytes, 64 * 1024);
   if ((e = hipHostMalloc(&s.host, cap, hipHostMallocDefault)) != hipSuccess) return hip_status(e, "hipHostMalloc");
-  if ((e = hipMalloc(&s.dev, cap)) != hipSuccess) return hip_status(e, "hipMalloc(group table)");
+  if ((e = dev_alloc(&s.dev, cap)) != hipSuccess) {
+    s.dev = nullptr;
+    return hip_status(e, "hipMalloc(group table)");
+  }
   s.cap = cap;
   return FG_OK;
 }

@@ -333,7 +333,7 @@ int fg_server_add_weights(fg_server *S, int dtype, int f32_mode, int num_gemm, c
     // pack once on the first server stream; registration waits for it
     hipStream_t st = S->streams[0];
     for (int i = 0; i < num_gemm && !rc; ++i) {
-      rc = hip_status(hipMalloc(&w.packed[i], fg_packed_weight_bytes(dtype, f32_mode, M[i], K[i])),
+      rc = hip_status(dev_alloc(&w.packed[i], fg_packed_weight_bytes(dtype, f32_mode, M[i], K[i])),
                       "hipMalloc(packed weights)");
       if (!rc) rc = fg_pack_weight(dtype, f32_mode, A[i], M[i], K[i], w.packed[i], st);
     }

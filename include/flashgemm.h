@@ -334,6 +334,15 @@ int fg_to_f64(int x_kind, const void *x, double *out, long n, void *stream);
  * out is a host pointer; the call synchronizes the stream. */
 int fg_error_norms(int x_kind, const void *x, const double *ref, long n, double *out, void *stream);
 
+/* ---- Allocation-failure injection (tests of the error paths; no counterpart in the
+ * reference, whose calls abort on a failed posix_memalign) ----
+ * The next `count` device allocations the library makes (per-stream scratch, grouped
+ * launch tables, autotune buffers, a server's packed weight copies) fail as if the
+ * device were out of memory: the call that needed one returns FG_ERR_NOMEM and leaves
+ * nothing half-built, so the same call succeeds once the injection is spent.
+ * count <= 0 clears it. Returns FG_OK. */
+int fg_inject_alloc_failures(int count);
+
 /* ---- L2 -> CU stream rate (measurement only; no counterpart in the reference) ----
  * `workgroups` workgroups of 8 waves; wave w of each streams its own slice_kb KB slice
  * of a region the workgroups share (8 slices), 1 KB per load (16 B per lane), 16

@@ -44,7 +44,7 @@ template <int KIND> struct KT;
 // KC: k consumed per MFMA step (both lane halves). RB/CW: rows x columns of B one
 // thread stages per stage.
 template <> struct KT<K_F32> {
-  static constexpr int BK = 32, KG = 4, ESZ = 4, KC = 8, RB = 4, CW = 4;
+  static constexpr int BK = 32, KG = 4, ESZ = 4, KC = 16, RB = 4, CW = 4;
   using Acc = f32x16;
 };
 template <> struct KT<K_BF16> {
@@ -326,6 +326,37 @@ __device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int k
       }
     return;
   }
+  if constexpr (KIND == K_F32) {
+    // 16x16x4 MFMAs (the f32 shape with the higher issue rate on gfx950: 156 against 148
+    // TFLOP/s from registers): lane quad q holds k = kc + 4q + s (s = 0..3, one 16-byte
+    // group) of row (column) lane % 16 of each 16-row (16-column) sub-block; sub-block
+    // (ii, jj) of 32 x 32 block (i, j) accumulates in elements 4 (2 ii + jj) .. + 3 of
+    // acc[i][j]. Every accumulator gets its MFMAs in k order, 4 JN others between them.
+    const int q = lane >> 4, l16 = lane & 15;
+    const int g4 = kc / T::KG + q;
+    const int rr = wm * (BT / TL::WM) + l16, cc = wn * (BT / TL::WN) + l16;
+    // (row blocks one at a time, so the 256 x 256 tile's fragments fit its registers)
+    f32x4 bv[2 * JN];
+#pragma unroll
+    for (int j = 0; j < 2 * JN; ++j) bv[j] = __builtin_bit_cast(f32x4, sB[slot<BT>(g4, cc + j * 16)]);
+#pragma unroll
+    for (int i = 0; i < IM; ++i) {
+      const f32x4 av[2] = {__builtin_bit_cast(f32x4, sA[slot<BT>(g4, rr + i * 32)]),
+                           __builtin_bit_cast(f32x4, sA[slot<BT>(g4, rr + i * 32 + 16)])};
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int j = 0; j < JN; ++j)
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            f32x4 c = {acc[i][j][4 * sb], acc[i][j][4 * sb + 1], acc[i][j][4 * sb + 2], acc[i][j][4 * sb + 3]};
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[sb >> 1][s], bv[2 * j + (sb & 1)][s], c, 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) acc[i][j][4 * sb + t] = c[t];
+          }
+    }
+    return;
+  }
   const int g = kc / T::KG + h;
   uint4 a[IM], b[JN];
 #pragma unroll
@@ -508,35 +539,44 @@ __device__ __forceinline__ void gemm_tile(const void *__restrict__ Av, const voi
     }
   };
   char *const Cb = (char *)Cv + (EPI == EPI_PARTIAL ? (size_t)slab * M * N * CE : 0);
+  // element r of a 32 x 32 block's accumulator, for this lane: row lrow + row_of(r),
+  // column lcol + col_of(r) of the block. 32x32 MFMAs (BF16, INT8): rows 8(r/4) + 4h +
+  // (r%4) of column l32; FP32's 16x16 sub-blocks: rows 16 (r/8) + 4q + (r%4) of column
+  // 16 ((r/4) % 2) + lane % 16.
+  constexpr bool S16 = KIND == K_F32;
+  const int lrow = S16 ? 4 * (lane >> 4) : 4 * h, lcol = S16 ? (lane & 15) : l32;
+  auto row_of = [](int r) { return S16 ? 16 * (r >> 3) + (r & 3) : 8 * (r >> 2) + (r & 3); };
+  auto col_of = [](int r) { return S16 ? 16 * ((r >> 2) & 1) : 0; };
   if (m0 + BT <= M && n0 + BT <= N) {
     // whole tile inside C: one lane pointer per block, rows at uniform strides
     const size_t rs = (size_t)N * CE;
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
-      const long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
+      const long n = n0 + wn * (BT / TL::WN) + j * 32 + lcol;
       int bias = 0;
       if (KIND == K_I8 && (EPI != EPI_PARTIAL || !colsum)) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
 #pragma unroll
       for (int i = 0; i < IM; ++i) {
-        char *const p = Cb + ((size_t)(m0 + wm * (BT / TL::WM) + i * 32 + 4 * h) * N + n) * CE;
+        char *const p = Cb + ((size_t)(m0 + wm * (BT / TL::WM) + i * 32 + lrow) * N + n) * CE;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) put(p + (size_t)(8 * (r >> 2) + (r & 3)) * rs, acc[i][j], r, bias);
+        for (int r = 0; r < 16; ++r)
+          put(p + (size_t)row_of(r) * rs + (size_t)col_of(r) * CE, acc[i][j], r, bias);
       }
     }
     return;
   }
 #pragma unroll
   for (int j = 0; j < JN; ++j) {
-    long n = n0 + wn * (BT / TL::WN) + j * 32 + l32;
-    if (n >= N) continue;
+    const long nb = n0 + wn * (BT / TL::WN) + j * 32 + lcol;
     int bias = 0;
-    if (KIND == K_I8 && (EPI != EPI_PARTIAL || !colsum)) bias = 128 * (colsum ? colsum[n] : csum_tile[n - n0]);
+    if (KIND == K_I8 && (EPI != EPI_PARTIAL || !colsum) && nb < N)
+      bias = 128 * (colsum ? colsum[nb] : csum_tile[nb - n0]);
 #pragma unroll
     for (int i = 0; i < IM; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        long m = m0 + wm * (BT / TL::WM) + i * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        if (m >= M) continue;
+        const long m = m0 + wm * (BT / TL::WM) + i * 32 + lrow + row_of(r), n = nb + col_of(r);
+        if (m >= M || n >= N) continue;
         put(Cb + ((size_t)m * N + n) * CE, acc[i][j], r, bias);
       }
   }

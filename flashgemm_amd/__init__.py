@@ -161,6 +161,7 @@ def lib() -> ctypes.CDLL:
         L.fg_server_run_trace.argtypes = [vp, i, ctypes.POINTER(i), lp, vpp, vpp, ctypes.POINTER(ctypes.c_double),
                                           ctypes.c_double]
         L.fg_server_latencies.argtypes = [vp, l, l, ctypes.POINTER(ctypes.c_double)]
+        L.fg_server_cancel.argtypes = [vp, l]
         L.fg_server_num_requests.argtypes = [vp]
         L.fg_server_num_requests.restype = l
         L.fg_server_destroy.argtypes = [vp]
@@ -511,6 +512,15 @@ class Server:
 
     def drain(self, timeout_s: float = 600.0) -> None:
         _check(lib().fg_server_drain(self.h, timeout_s))
+
+    def cancel(self, req_id: int) -> bool:
+        """True when the request was dropped before admission (its C is never written);
+        False when it was already admitted, running or done (it completes normally)."""
+        rc = lib().fg_server_cancel(self.h, req_id)
+        if rc == ERR_INVALID and req_id >= 0 and req_id < lib().fg_server_num_requests(self.h):
+            return False
+        _check(rc)
+        return True
 
     def run_trace(self, set_ids, Ns, Bs, Cs, arrivals, timeout_s: float = 600.0) -> None:
         n = len(set_ids)

@@ -52,6 +52,7 @@ struct Request {
   size_t bytes;
   Clock::time_point arrival, done;
   int status;
+  bool cancelled;
 };
 
 // The requests of one part of a launched batch, completed when `ev` has: the on-chip
@@ -412,6 +413,22 @@ int fg_server_drain(fg_server *S, double timeout_s) {
   return FG_OK;
 }
 
+int fg_server_cancel(fg_server *S, long req_id) {
+  if (!S) return set_error(FG_ERR_INVALID, "cancel");
+  std::lock_guard<std::mutex> lk(S->mu);
+  if (req_id < 0 || req_id >= (long)S->reqs.size()) return set_error(FG_ERR_INVALID, "cancel: no request %ld", req_id);
+  // only a request still waiting for admission holds nothing (no slot, no budget, no
+  // launch): the scheduler takes requests off `pending` under this lock
+  auto it = std::find(S->pending.begin(), S->pending.end(), req_id);
+  if (it == S->pending.end())
+    return set_error(FG_ERR_INVALID, "cancel: request %ld is already admitted, running or done", req_id);
+  S->pending.erase(it);
+  S->reqs[req_id].cancelled = true;
+  S->completed += 1;  // drain counts it as settled
+  S->cv.notify_all();
+  return FG_OK;
+}
+
 int fg_server_run_trace(fg_server *S, int n, const int *set_ids, const long *Ns, const void *const *Bs,
                         void *const *Cs, const double *arrival_s, double timeout_s) {
   if (!S || n < 0) return set_error(FG_ERR_INVALID, "run_trace");
@@ -435,7 +452,7 @@ int fg_server_latencies(fg_server *S, long first, long n, double *out) {
   if (first + n > (long)S->reqs.size()) return set_error(FG_ERR_INVALID, "latencies: only %zu requests", S->reqs.size());
   for (long i = 0; i < n; ++i) {
     const Request &r = S->reqs[first + i];
-    out[i] = (r.done.time_since_epoch().count() == 0) ? -1.0 : seconds(r.arrival, r.done);
+    out[i] = r.cancelled ? -2.0 : (r.done.time_since_epoch().count() == 0) ? -1.0 : seconds(r.arrival, r.done);
   }
   return FG_OK;
 }

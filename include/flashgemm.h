@@ -300,7 +300,13 @@ int fg_server_drain(fg_server *srv, double timeout_s);
 /* Submit n requests at the given arrival offsets (seconds from the call), then drain. */
 int fg_server_run_trace(fg_server *srv, int n, const int *set_ids, const long *Ns, const void *const *Bs,
                         void *const *Cs, const double *arrival_s, double timeout_s);
-/* Latency (completion - arrival, seconds) of requests [first, first + n). */
+/* Cancel request req_id while it still waits for a slot or budget (not yet admitted to
+ * a launch): it is dropped, its C is never written, and fg_server_drain no longer
+ * waits for it. FG_OK when cancelled; FG_ERR_INVALID for an unknown id or a request
+ * already admitted, running or done (it then completes normally). */
+int fg_server_cancel(fg_server *srv, long req_id);
+/* Latency (completion - arrival, seconds) of requests [first, first + n); -1 for a
+ * request not completed yet, -2 for a cancelled one. */
 int fg_server_latencies(fg_server *srv, long first, long n, double *out);
 long fg_server_num_requests(fg_server *srv);
 void fg_server_destroy(fg_server *srv);

@@ -102,3 +102,49 @@ def test_server_weight_registration_failure_releases_budget(gpu, injection):
         assert_rel(C.cpu().numpy(), O.chain_bf16(B, As, O.LAYOUT_MATH), TOL_BF16)
     finally:
         srv.close()
+
+
+def test_server_cancel_waiting_requests(gpu):
+    """One request slot held by a long chain (about 4 ms of GPU work) while nine short
+    requests wait for admission: the last five are cancelled. Cancelled requests leave
+    C untouched, report latency -2 and do not hold up drain; the others complete and
+    match the oracle. Ids that are unknown or already done are refused."""
+    import torch
+    print("server cancellation of waiting requests", flush=True)
+    rng = np.random.default_rng(94)
+    dtype = fg.DTYPE_BF16
+    big_dims, dims = [2048, 2048, 2048, 2048], [256, 512, 256]
+    _, big_As = rand_chain(rng, dtype, big_dims[0], 1, big_dims[1:])
+    _, As = rand_chain(rng, dtype, dims[0], 1, dims[1:])
+    srv = fg.Server(max_inflight=1, num_streams=1, use_graph=False)
+    try:
+        big_sid = srv.add_weights(dtype, [dev(a) for a in big_As])
+        sid = srv.add_weights(dtype, [dev(a) for a in As])
+        N0 = 131072
+        big_B = torch.randn((big_dims[0], N0), device="cuda").to(torch.bfloat16)
+        big_C = torch.zeros((big_dims[-1], N0), dtype=torch.float32, device="cuda")
+        small = []
+        for _ in range(9):
+            B, _ = rand_chain(rng, dtype, dims[0], 64, [1])
+            small.append((B, dev(B), torch.zeros((dims[-1], 64), dtype=torch.float32, device="cuda")))
+        torch.cuda.synchronize()
+        ids = [srv.submit(big_sid, big_B, big_C)] + [srv.submit(sid, dB, C) for _, dB, C in small]
+        cancelled = [srv.cancel(r) for r in ids[5:]]
+        srv.drain()
+        assert all(cancelled), cancelled  # the slot is held for milliseconds
+        lat = srv.latencies()
+        for i, (B, _, C) in enumerate(small, start=1):
+            got = C.cpu().numpy()
+            if i >= 5:
+                assert lat[i] == -2.0 and not got.any()
+            else:
+                assert lat[i] > 0
+                assert_rel(got, O.chain_bf16(B, As, O.LAYOUT_MATH), TOL_BF16)
+        assert lat[0] > 0
+        cols = big_B[:, :16].contiguous().view(torch.int16).cpu().numpy().view(np.uint16)  # bf16 bits
+        assert_rel(big_C[:, :16].cpu().numpy(), O.chain_bf16(cols, big_As, O.LAYOUT_MATH), TOL_BF16)
+        assert srv.cancel(ids[0]) is False  # done
+        with pytest.raises(fg.FlashgemmError):
+            srv.cancel(10 ** 6)
+    finally:
+        srv.close()

@@ -14,7 +14,7 @@
 //     operand layout: 16-byte k-groups per row (A) or per column (B), slot index
 //     g*rows + (r ^ (g & 7)) so both the staging writes and the fragment reads
 //     hit distinct banks;
-//   * kinds: FP32 (v_mfma_f32_32x32x2_f32), BF16 (v_mfma_f32_32x32x16_bf16),
+//   * kinds: FP32 (v_mfma_f32_16x16x4_f32), BF16 (v_mfma_f32_32x32x16_bf16),
 //     INT8 (v_mfma_i32_32x32x32_i8 with the u8 bias identity
 //     u8(a).b = s8(a ^ 0x80).b + 128.b, the 128.sum_k b term added in the
 //     epilogue), and FP32 split into bf16 hi/lo planes with three products.
@@ -367,14 +367,7 @@ __device__ __forceinline__ void mma_step(const uint4 *sA, const uint4 *sB, int k
   for (int i = 0; i < IM; ++i)
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
-      if (KIND == K_F32) {
-        // lane half h holds k = kc + 4h + s for s = 0..3 in both operands
-        f32x4 av = __builtin_bit_cast(f32x4, a[i]);
-        f32x4 bv = __builtin_bit_cast(f32x4, b[j]);
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s], bv[s], acc[i][j], 0, 0, 0);
-      } else if (KIND == K_BF16) {
+      if (KIND == K_BF16) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a[i]),
                                                             __builtin_bit_cast(bf16x8, b[j]), acc[i][j], 0, 0, 0);
       } else {

@@ -275,7 +275,8 @@ def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0, 40000.0), seed
     (continuous batching: requests that fit on chip in the grouped persistent kernel
     replayed from a hipGraph, the others in one grouped launch per chain level) and
     through the per-GEMM baseline (each request alone, one launch per GEMM), at 5k,
-    20k and 40k requests/s offered."""
+    20k and 40k requests/s offered, each mode on one server warmed up by the trace's
+    first 1,000 requests."""
     import numpy as np
     sets = W.serving_sets(torch, seed=seed)
     set_of, Ns, gaps = W.serving_trace(n_req, len(sets), seed=seed + 1)
@@ -306,26 +307,37 @@ def bench_serving(torch, fg, n_req=10000, rates=(5000.0, 20000.0, 40000.0), seed
 
     refs = {int(i): f64_ref(i) for i in sample if sets[set_of[i]]["dtype"] != fg.DTYPE_INT8}
     torch.cuda.synchronize()
-    out = dict(requests=n_req, weight_sets=len(sets), total_gflop=flop / 1e9)
-    for qi, qps in enumerate(rates):
-        arrivals = (np.cumsum(gaps) / qps).tolist()
-        r = {}
-        kept = {}
-        for mode in ("engine", "per_gemm_baseline"):
-            srv = fg.Server(per_gemm_baseline=(mode != "engine"))
-            try:
-                sids = [srv.add_weights(s["dtype"], s["As"]) for s in sets]
+    # One server per mode serves every rate, after an untimed warm-up pass over the first
+    # `warm` requests of the trace: the measured runs see steady state, not the one-time
+    # costs of a fresh server (the first launch of each kernel variant loading its code,
+    # the first scratch and table allocations of its streams), which otherwise set the
+    # p99 of the first rate (the slowest requests were the trace's first hundred).
+    warm = min(1000, n_req)
+    out = dict(requests=n_req, weight_sets=len(sets), total_gflop=flop / 1e9, warmup_requests=warm)
+    res = {q: {} for q in rates}
+    kept = {}
+    for mode in ("engine", "per_gemm_baseline"):
+        srv = fg.Server(per_gemm_baseline=(mode != "engine"))
+        try:
+            sids = [srv.add_weights(s["dtype"], s["As"]) for s in sets]
+            ids = [sids[i] for i in set_of]
+            nn = [int(n) for n in Ns]
+            srv.run_trace(ids[:warm], nn[:warm], Bs[:warm], Cs[:warm], (np.cumsum(gaps[:warm]) / rates[0]).tolist())
+            for qi, qps in enumerate(rates):
+                first = srv.num_requests()
                 t0 = time.perf_counter()
-                srv.run_trace([sids[i] for i in set_of], [int(n) for n in Ns], Bs, Cs, arrivals)
+                srv.run_trace(ids, nn, Bs, Cs, (np.cumsum(gaps) / qps).tolist())
                 wall = time.perf_counter() - t0
-                lat = np.array(srv.latencies()) * 1e3
-            finally:
-                srv.close()
-            r[mode] = dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
-                           mean_ms=float(lat.mean()), requests_per_s=n_req / wall, tflops=flop / wall / 1e12,
-                           wall_s=wall)
-            if qi == 0:
-                kept[mode] = {int(i): c_view(i).clone() for i in sample}
+                lat = np.array(srv.latencies(first, n_req)) * 1e3
+                res[qps][mode] = dict(p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                                      mean_ms=float(lat.mean()), requests_per_s=n_req / wall,
+                                      tflops=flop / wall / 1e12, wall_s=wall)
+                if qi == 0:
+                    kept[mode] = {int(i): c_view(i).clone() for i in sample}
+        finally:
+            srv.close()
+    for qi, qps in enumerate(rates):
+        r = res[qps]
         r["p99_speedup_vs_per_gemm"] = r["per_gemm_baseline"]["p99_ms"] / r["engine"]["p99_ms"]
         if qi == 0:
             r["verified"] = verify_serving(torch, fg, sets, set_of, Ns, sample, kept, refs)

@@ -529,6 +529,10 @@ class Server:
                                          (ctypes.c_void_p * n)(*[_addr(c) for c in Cs]),
                                          (ctypes.c_double * n)(*arrivals), timeout_s))
 
+    def num_requests(self) -> int:
+        """Requests submitted to this server so far (warm-up and traces included)."""
+        return lib().fg_server_num_requests(self.h)
+
     def latencies(self, first: int = 0, n: int | None = None):
         total = lib().fg_server_num_requests(self.h)
         n = total - first if n is None else n

@@ -4,7 +4,8 @@
 // kernels (singlegemm/src/kernel.h, kernelMGN.h, PACK_x86.h, PACK_B.h;
 // multigemm/src/kernel_*.h) with one workgroup-level design:
 //   * two tile shapes: 256 threads = 4 waves in a 2 x 2 grid on a 128 x 128 output
-//     tile (each wave 64 x 64 = 2 x 2 MFMA 32x32 blocks), and for large problems
+//     tile (each wave 64 x 64 = 2 x 2 blocks of 32 x 32, one MFMA tile each, or four
+//     16 x 16 ones for FP32), and for large problems
 //     512 threads = 8 waves in a 2 x 4 grid on a 256 x 256 tile (each wave
 //     128 x 64 = 4 x 2 blocks), which halves the operand bytes per MFMA; a single
 //     FP32 GEMM with at most one 128 x 128 tile per CU (2048^3) runs each tile with
@@ -12,8 +13,9 @@
 //     the halves added in a fixed order through LDS;
 //   * A and B staged global -> registers -> LDS, double buffered, in the MFMA
 //     operand layout: 16-byte k-groups per row (A) or per column (B), slot index
-//     g*rows + (r ^ (g & 7)) so both the staging writes and the fragment reads
-//     hit distinct banks;
+//     g*rows + (r ^ (g & 7)) so the staging writes and the 32-row fragment reads hit
+//     distinct banks (FP32's 16-row reads of four k-groups at once take 4 conflict
+//     cycles each, which the 16x16x4 issue rate more than repays);
 //   * kinds: FP32 (v_mfma_f32_16x16x4_f32), BF16 (v_mfma_f32_32x32x16_bf16),
 //     INT8 (v_mfma_i32_32x32x32_i8 with the u8 bias identity
 //     u8(a).b = s8(a ^ 0x80).b + 128.b, the 128.sum_k b term added in the

@@ -12,10 +12,9 @@
 //     two groups of 4 waves splitting its k stages (512 threads, two waves per SIMD),
 //     the halves added in a fixed order through LDS;
 //   * A and B staged global -> registers -> LDS, double buffered, in the MFMA
-//     operand layout: 16-byte k-groups per row (A) or per column (B), slot index
-//     g*rows + (r ^ (g & 7)) so the staging writes and the 32-row fragment reads hit
-//     distinct banks (FP32's 16-row reads of four k-groups at once take 4 conflict
-//     cycles each, which the 16x16x4 issue rate more than repays);
+//     operand layout: 16-byte k-groups per row (A) or per column (B), placed by a
+//     row and group permutation (slot()) under which the staging writes and the
+//     fragment reads hit distinct banks;
 //   * kinds: FP32 (v_mfma_f32_16x16x4_f32), BF16 (v_mfma_f32_32x32x16_bf16),
 //     INT8 (v_mfma_i32_32x32x32_i8 with the u8 bias identity
 //     u8(a).b = s8(a ^ 0x80).b + 128.b, the 128.sum_k b term added in the
@@ -64,16 +63,18 @@ template <> struct KT<K_F32X3> {
 
 // Operand tile per stage: BK/KG groups x BT rows x 16 B, or for F32X3 two planes
 // (hi, lo) of BK/8 groups x BT rows x 16 B: 8 x BT 16-byte slots either way.
-// Row r of group g sits at g*BT + (f(r) ^ (g & 7)) with
+// Row r of group g sits at g*BT + (f(r) ^ p(g)) with
 // f(r) = r ^ 4(r & 1) ^ ((r >> 3) & 1) ^ 2((r >> 4) & 1), a permutation within every
-// aligned 32 rows chosen so that eight 16-byte slots (one LDS cycle of 128 bytes) land
-// on eight different bank groups for every access pattern here: 8 consecutive rows of
-// one group (fragment reads), one row in 8 groups (A staging), rows r and r+1 in 4
-// groups (split-bf16 A staging, 8-byte halves), every second and every fourth column
-// of one group (B staging with 2 or 4 columns per thread).
+// aligned 32 rows, and p(g) = 2(g & 3) + ((g >> 2) & 1), a permutation of the low three
+// bits by group. Measured bank-conflict free (SQ_LDS_BANK_CONFLICT, a probe of each
+// pattern) for every access here: 32 consecutive rows of two groups (32x32 fragment
+// reads), 16 consecutive rows of four groups (FP32's 16x16 fragment reads; with
+// p(g) = g & 7 these took 4 conflict cycles per read), one row in 8 groups (A staging),
+// rows r and r+1 in 4 groups (split-bf16 A staging, 8-byte halves), every second and
+// every fourth column of one group (B staging with 2 or 4 columns per thread).
 template <int BT>
 __device__ __forceinline__ int slot(int g, int r) {
-  return g * BT + ((r ^ ((r & 1) << 2) ^ ((r >> 3) & 1) ^ (((r >> 4) & 1) << 1)) ^ (g & 7));
+  return g * BT + ((r ^ ((r & 1) << 2) ^ ((r >> 3) & 1) ^ (((r >> 4) & 1) << 1)) ^ (((g & 3) << 1) | ((g >> 2) & 1)));
 }
 
 // ---------------------------------------------------------------- staging --

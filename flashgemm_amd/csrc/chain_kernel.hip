@@ -47,8 +47,10 @@ template <int KIND> struct CT;
 // KA: A elements per lane-half per step; ESZ: global element bytes; RP: row blocks
 // per wave pass;
 // PF: k-steps per prefetch chunk; FS: packed bytes per lane per fragment.
+// FP32 runs 16x16x4 MFMAs (measured 99% of their issue rate from registers, against
+// 94% for 32x32x2): a k-step is 16 k, its fragment two 16-row halves.
 template <> struct CT<K_F32> {
-  static constexpr int KG = 4, KC = 8, KA = 4, ESZ = 4, RP = 1, PF = 8, FS = 16;
+  static constexpr int KG = 4, KC = 16, KA = 4, ESZ = 4, RP = 1, PF = 4, FS = 32;
   using Acc = f32x16;
 };
 template <> struct CT<K_BF16> {
@@ -82,7 +84,9 @@ __device__ __forceinline__ int fresh_tid() {
 // for row group rg (RP * 32 rows), k-step ks < kspad and row block i; fragment f is
 // 64 lanes x FS bytes at f * 64 * FS, lane l = h*32 + r holding row
 // (rg*RP + i)*32 + r, elements k = ks*KC + h*KA .. + KA-1 (zero outside A). For
-// F32X3 the fragment is the 1 KB hi half then the 1 KB lo half.
+// F32X3 the fragment is the 1 KB hi half then the 1 KB lo half. For F32 it is two
+// 1 KB halves of 16 rows: lane l = q*16 + r of half a holds row rg*32 + 16a + r,
+// elements k = ks*16 + 4q .. + 3.
 template <int KIND>
 __host__ __device__ inline long kspad_of(long K) {
   using T = CT<KIND>;
@@ -102,6 +106,31 @@ template <int KIND>
 __device__ __forceinline__ void pack_frag(const char *__restrict__ A, long M, long K, char *__restrict__ out, long f,
                                           int lane) {
   using T = CT<KIND>;
+  if constexpr (KIND == K_F32) {
+    const int q = lane >> 4, r16 = lane & 15;
+    const unsigned kspad = (unsigned)kspad_of<KIND>(K);
+    const unsigned fu = (unsigned)f, rg = fu / kspad, ks = fu - rg * kspad;
+    const long k0 = (long)ks * 16 + 4 * q;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const long row = (long)rg * 32 + 16 * a + r16;
+      uint32_t w[4] = {0u, 0u, 0u, 0u};
+      const char *src = A + ((size_t)row * K + k0) * 4;
+      if (row < M && k0 + 4 <= K && ((uintptr_t)src & 15) == 0) {
+        const uint4 v = *(const uint4 *)src;
+        w[0] = v.x;
+        w[1] = v.y;
+        w[2] = v.z;
+        w[3] = v.w;
+      } else if (row < M) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if (k0 + e < K) w[e] = *(const uint32_t *)(src + 4 * e);
+      }
+      *(uint4 *)(out + f * 64 * T::FS + a * 1024 + lane * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    return;
+  }
   const int h = lane >> 5, r = lane & 31;
   // 32-bit index math: a packed matrix has far fewer than 2^31 fragments, and a
   // 64-bit division is a long software sequence on the GPU
@@ -163,23 +192,44 @@ __global__ __launch_bounds__(256) void pack_a_kernel(PackJob J) {
   pack_frag<KIND>((const char *)J.A[m], J.M[m], J.K[m], (char *)J.out[m], f - J.fstart[m], threadIdx.x & 63);
 }
 
-// One lane's packed A fragment: 16 bytes, or hi and lo halves for F32X3.
-template <int KIND> struct AFrag { uint4 v[KIND == K_F32X3 ? 2 : 1]; };
+// One lane's packed A fragment: 16 bytes, or hi and lo halves for F32X3, or the two
+// 16-row halves for F32.
+template <int KIND> struct AFrag { uint4 v[KIND == K_F32X3 || KIND == K_F32 ? 2 : 1]; };
 
 // One lane's B fragments for one k-step: TNB column blocks (hi and lo planes for
-// F32X3).
+// F32X3; for F32 two 16-column halves per block, b[2j + half]).
 template <int KIND, int TNB> struct BFrag {
-  uint4 b[TNB];
+  uint4 b[KIND == K_F32 ? 2 * TNB : TNB];
   uint4 lo[KIND == K_F32X3 ? TNB : 1];
 };
 
-// FP32 alternates its four MFMAs of a step between acc and acc2 (summed when the
-// pass ends), so consecutive MFMAs of a wave never wait on each other's result.
+// One k-step of MFMAs. FP32: 16x16x4 MFMAs, k sub-step s outermost, so the
+// 4 TNB products between two MFMAs on one accumulator hide the MFMA latency; lane
+// q*16 + l of half a of the A fragment holds row 16a + l, of b[2j + b] column
+// 16b + l of block j, both at k = 4q + s, and sub-block (a, b) of block j is acc
+// elements 4(2a + b) .. + 3.
 template <int KIND, int TNB>
 __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KIND> (&af)[CT<KIND>::RP],
-                                      typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB],
-                                      typename CT<KIND>::Acc (&acc2)[CT<KIND>::RP][TNB]) {
+                                      typename CT<KIND>::Acc (&acc)[CT<KIND>::RP][TNB]) {
   constexpr int RP = CT<KIND>::RP;
+  if constexpr (KIND == K_F32) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int j = 0; j < TNB; ++j)
+#pragma unroll
+        for (int i = 0; i < RP; ++i)
+#pragma unroll
+          for (int sb = 0; sb < 4; ++sb) {
+            const f32x4 av = __builtin_bit_cast(f32x4, af[i].v[sb >> 1]);
+            const f32x4 bv = __builtin_bit_cast(f32x4, bf.b[2 * j + (sb & 1)]);
+            f32x4 c = {acc[i][j][4 * sb], acc[i][j][4 * sb + 1], acc[i][j][4 * sb + 2], acc[i][j][4 * sb + 3]};
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[s], bv[s], c, 0, 0, 0);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) acc[i][j][4 * sb + e] = c[e];
+          }
+    return;
+  }
 #pragma unroll
   for (int j = 0; j < TNB; ++j) {
 #pragma unroll
@@ -191,15 +241,6 @@ __device__ __forceinline__ void mma_k(const BFrag<KIND, TNB> &bf, const AFrag<KI
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[i][j], 0, 0, 0);
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[i][j], 0, 0, 0);
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[i][j], 0, 0, 0);
-      } else if (KIND == K_F32) {
-        // lane half h holds k = kc + 4h + s for s = 0..3 in both operands
-        const f32x4 av = __builtin_bit_cast(f32x4, af[i].v[0]);
-        const f32x4 bv = __builtin_bit_cast(f32x4, bf.b[j]);
-#pragma unroll
-        for (int s = 0; s < 4; s += 2) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s], bv[s], acc[i][j], 0, 0, 0);
-          acc2[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s + 1], bv[s + 1], acc2[i][j], 0, 0, 0);
-        }
       } else if (KIND == K_BF16) {
         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, af[i].v[0]),
                                                             __builtin_bit_cast(bf16x8, bf.b[j]), acc[i][j], 0, 0, 0);
@@ -527,8 +568,10 @@ __device__ __forceinline__ void store_rows_refl(const ChainParams &P, int gi, lo
       for (int e = 0; e < 4; ++e) atomicAdd(&csum[(cb0 + j) * 32 + 8 * e + (l32 >> 2)], cs[j][e]);
 }
 
-// Epilogue of one pass: lane holds rows 8(r/4) + 4h + (r%4) of column j*32 + l32 of
-// row block rb0 + 32 i. The last GEMM writes C, the others Y_gi into the arena (INT8
+// Epilogue of one pass: element r of acc[i][j] is row lrow + crow(r) and column
+// lcol + ccol(r) of the 32 x 32 block (rb0 + 32 i, column block j): for the 32x32
+// MFMAs rows 4h + 8(r/4) + (r%4) of column l32, for FP32's 16x16 MFMAs (sub-block
+// r/4 = 2a + b) rows 4q + 16a + (r%4) of column l16 + 16b, q = lane / 16. The last GEMM writes C, the others Y_gi into the arena (INT8
 // also adds the s8 column sums of what it writes into csum: the next GEMM's bias).
 // Index math is 32-bit (arena rows and M fit easily) and the row-bound checks are
 // only evaluated in the block that straddles M. EPX applies GEMM gi's epilogue
@@ -545,6 +588,9 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
   const int lo_off = (KIND == K_F32X3) ? SG * TN : 0;
   const int lane = fresh_tid() & 63;
   const int h = lane >> 5, l32 = lane & 31;
+  const int lrow = KIND == K_F32 ? 4 * (lane >> 4) : 4 * h, lcol = KIND == K_F32 ? (lane & 15) : l32;
+  auto crow = [](int r) { return KIND == K_F32 ? 16 * (r >> 3) + (r & 3) : 8 * (r >> 2) + (r & 3); };
+  auto ccol = [](int r) { return KIND == K_F32 ? 16 * ((r >> 2) & 1) : 0; };
   const int M = (int)P.M[gi], Mp = (M + 31) & ~31;
   const int rb0 = (int)rb0_l;
   const EpiOne ex = EPX ? P.epi.g[gi] : EpiOne{};
@@ -596,22 +642,21 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
       const bool full = rbase + 32 <= M;
 #pragma unroll
       for (int j = 0; j < TNW; ++j) {
-        const int n = (cb0 + j) * 32 + l32;
-        if (n >= tile_w) continue;
-        char *base = (char *)P.C + ((size_t)(rbase + 4 * h) * P.N + n0 + n) * 4;
+        const int n = (cb0 + j) * 32 + lcol;
+        char *base = (char *)P.C + ((size_t)(rbase + lrow) * P.N + n0 + n) * 4;
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int ro = 8 * (r >> 2) + (r & 3);
-          const int m = rbase + 4 * h + ro;
-          if (full || m < M) {
+          const int m = rbase + lrow + crow(r);
+          if ((full || m < M) && n + ccol(r) < tile_w) {
+            char *const dst = base + crow(r) * rs + ccol(r) * 4;
             if (KIND == K_I8) {
               int v = (int)acc[i][j][r];
               if (EPX) v = epi_i32(v, (int)lb[m], ex.relu);
-              __builtin_nontemporal_store(v, (int *)(base + ro * rs));
+              __builtin_nontemporal_store(v, (int *)dst);
             } else {
               float v = (float)acc[i][j][r];
               if (EPX) v = epi_f(v, has_scale, ls[m], has_bias, __builtin_bit_cast(float, lb[m]), ex.relu);
-              __builtin_nontemporal_store(v, (float *)(base + ro * rs));
+              __builtin_nontemporal_store(v, (float *)dst);
             }
           }
         }
@@ -625,14 +670,16 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
       return;
     }
   }
-  // rows rbase + 8q + 4h + (0..3), q = 0..3: four consecutive rows per q. Padded rows
-  // (m >= M) are stored as zero: they are the next GEMM's k-padding. A 32-row block
-  // starts on a multiple of 32 and S is a multiple of a chunk, so no block wraps: its
-  // slots are a wave-uniform block base, a per-lane offset (column l32, lane half h)
-  // and a compile-time offset per (q, j), which the LDS stores take as immediates.
-  // Row r of the block lies in group r / KG at byte (r % KG) * EB of its slot.
+  // rows rbase + lrow + crow(4q) + (0..3) of column lcol + ccol(4q), q = 0..3: four
+  // consecutive rows per q. Padded rows (m >= M) are stored as zero: they are the
+  // next GEMM's k-padding. A 32-row block starts on a multiple of 32 and S is a
+  // multiple of a chunk, so no block wraps: its slots are a wave-uniform block base,
+  // a per-lane offset (lrow, lcol) and a compile-time offset per (q, j), which the
+  // LDS stores take as immediates. Row r of the block lies in group r / KG at byte
+  // (r % KG) * EB of its slot (lrow and crow(4q) are multiples of 4, and of KG when
+  // KG is 4).
   constexpr int EB = (KIND == K_F32) ? 4 : (KIND == K_I8 ? 1 : 2);  // arena bytes per element (per plane)
-  const int lane_off = l32 * 16 + (T::KG == 4 ? h * TN * 16 : 4 * h * EB);
+  const int lane_off = lcol * 16 + (lrow / T::KG) * TN * 16 + (lrow % T::KG) * EB;
   int cs[TNW];
 #pragma unroll
   for (int j = 0; j < TNW; ++j) cs[j] = 0;
@@ -647,7 +694,7 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
     char *const blk = (char *)arena + (rs / T::KG) * TN * 16 + lane_off;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-      const int m0r = rbase + 8 * q + 4 * h;
+      const int m0r = rbase + lrow + crow(4 * q);
       const int valid = full ? 4 : (M - m0r < 0 ? 0 : (M - m0r > 4 ? 4 : M - m0r));
       // per-row extension operands of the 4 rows (rows past M use row M - 1)
       int eb[4];
@@ -663,7 +710,8 @@ __device__ __forceinline__ void store_rows_impl(const ChainParams &P, int gi, lo
       }
 #pragma unroll
       for (int j = 0; j < TNW; ++j) {
-        char *const dst = blk + ((8 * q / T::KG) * TN + (cb0 + j) * 32) * 16 + (8 * q % T::KG) * EB;
+        char *const dst =
+            blk + ((crow(4 * q) / T::KG) * TN + (cb0 + j) * 32 + ccol(4 * q)) * 16 + (crow(4 * q) % T::KG) * EB;
         if (KIND == K_I8) {
           int a4[4];
 #pragma unroll
@@ -763,9 +811,10 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   //
   // RAW (FP32 with every M_i % 32 == 0 and K_i a whole number of chunks): P.A[gi] are
   // the row-major weights themselves, read in the packed fragments' element order
-  // (lane h * 32 + r: row r of the block, k 4h..4h+3 of the step) at row stride K_gi,
-  // so the per-call pack is skipped; the lane offset then depends on the GEMM and
-  // travels with the chunk pointer (a chunk of the next GEMM is fetched early).
+  // (lane q * 16 + r of half a: row 16a + r of the block, k 4q..4q+3 of the step) at
+  // row stride K_gi, so the per-call pack is skipped; the lane offset and the byte
+  // offset of the second half (16 rows) then depend on the GEMM and travel with the
+  // chunk pointer (a chunk of the next GEMM is fetched early).
   constexpr bool RAW = RAWA && KIND == K_F32;
   static_assert(!RAW || RP == 1, "row-major weights: one row block per wave pass");
   auto first_ptr = [&](int gi) -> const char * {
@@ -774,8 +823,14 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     return (const char *)P.A[gi] + (size_t)wave * kspad_of<KIND>(P.K[gi]) * RP * FRAG;
   };
   const unsigned lane16 = (unsigned)lane * 16u;
-  auto lane_of = [&](int gi) -> unsigned {
-    return RAW ? (unsigned)(((long)l32 * P.K[gi] + 4 * h) * 4) : lane16;
+  // lane offset and second-half offset (wave-uniform) of GEMM gi's weight stream
+  struct Lane {
+    unsigned v;
+    int h1;
+  };
+  auto lane_of = [&](int gi) -> Lane {
+    if (RAW) return Lane{(unsigned)(((long)(lane & 15) * P.K[gi] + 4 * (lane >> 4)) * 4), (int)(16 * 4 * P.K[gi])};
+    return Lane{lane16, 1024};
   };
   constexpr size_t STEP = RAW ? (size_t)T::KC * 4 : (size_t)RP * FRAG;  // bytes per k-step
   AFrag<KIND> fa[2][PF][RP];
@@ -791,22 +846,23 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   // 64-column loop (256 workgroups, weights from L2) ran at 209 ns per k-step that way
   // against 158 ns with buffer loads. The record count only has to cover the chunk's
   // reads (offsets < 2 GiB), which are inside the weights.
-  auto load_step = [&](const char *base, unsigned lo, int s, AFrag<KIND>(&dst)[RP]) {
-    if (!RAW) lo = lane16;
+  auto load_step = [&](const char *base, Lane ln, int s, AFrag<KIND>(&dst)[RP]) {
+    const unsigned lo = RAW ? ln.v : lane16;
+    const int h1 = RAW ? ln.h1 : 1024;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, 0x7fffffff, BUF_DW3);
 #pragma unroll
     for (int i = 0; i < RP; ++i) {
       const int so = (int)((size_t)s * STEP + (size_t)i * FRAG);
       dst[i].v[0] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)lo, so, 0));
-      if (KIND == K_F32X3)
-        dst[i].v[1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)lo, so + 1024, 0));
+      if (KIND == K_F32X3 || KIND == K_F32)
+        dst[i].v[1] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(rs, (int)lo, so + h1, 0));
     }
   };
   // Issued strictly in step order: the loop waits for step s of a chunk with a count
   // of the loads issued after it, and a reordered burst (step 0 issued last) made the
   // wait analysis assume so on every later chunk too, a near-full drain of the stream
   // once per two chunks.
-  auto load_chunk = [&](const char *base, unsigned lo, AFrag<KIND>(&dst)[PF][RP]) {
+  auto load_chunk = [&](const char *base, Lane lo, AFrag<KIND>(&dst)[PF][RP]) {
 #pragma unroll
     for (int s = 0; s < PF; ++s) {
       load_step(base, lo, s, dst[s]);
@@ -816,20 +872,24 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
   constexpr size_t CHUNK = (size_t)PF * STEP;
 
   // B fragments of one k-step: column block j of this lane at slot
-  // (g0 + 2 s + h) * TN + 32 j + l32 for step s of the chunk whose first group is g0
-  const uint4 *lane_b = arena + h * TN + cb0 * 32 + l32;
+  // (g0 + 2 s + h) * TN + 32 j + l32 for step s of the chunk whose first group is g0;
+  // FP32 (four groups per step): 16-column half u of block j at slot
+  // (g0 + 4 s + q) * TN + 32 j + 16 u + l16, q = lane / 16, l16 = lane % 16
+  constexpr int GPS = T::KC / T::KG;  // arena groups per k-step
+  const uint4 *lane_b = KIND == K_F32 ? arena + (lane >> 4) * TN + cb0 * 32 + (lane & 15)
+                                      : arena + h * TN + cb0 * 32 + l32;
   auto read_b = [&](int g0, int s, BFrag<KIND, TNW> &dst) {
-    const uint4 *p = lane_b + (g0 + 2 * s) * TN;
+    const uint4 *p = lane_b + (g0 + GPS * s) * TN;
 #pragma unroll
-    for (int j = 0; j < TNW; ++j) {
-      dst.b[j] = p[32 * j];
+    for (int j = 0; j < (KIND == K_F32 ? 2 * TNW : TNW); ++j) {
+      dst.b[j] = p[(KIND == K_F32 ? 16 : 32) * j];
       if (KIND == K_F32X3) dst.lo[j] = p[lo_off + 32 * j];
     }
   };
 
   // invariant: at the start of each pass fa[0] holds (or has in flight) chunk 0
   const char *apk = first_ptr(0);
-  unsigned alane = lane_of(0);  // lane offset of the GEMM apk belongs to
+  Lane alane = lane_of(0);  // lane offsets of the GEMM apk belongs to
   const long K1c = (P.K[0] + CS - 1) / CS * CS;
   // A wave with no rows in GEMM 0 loads (and ignores) wave 0's first chunk, which
   // always exists: the loads stay unconditional and the wait counts after them exact.
@@ -887,9 +947,9 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
     const int ksteps = (int)((K + T::KC - 1) / T::KC);  // k-steps with real data
     const size_t pass_stride = RAW ? (size_t)NWR * RP * 32 * K * 4 : (size_t)NWR * kspad * RP * FRAG;
     const char *next_gemm = last ? nullptr : first_ptr(gi + 1);
-    const unsigned next_lane = last ? alane : lane_of(gi + 1);
+    const Lane next_lane = last ? alane : lane_of(gi + 1);
     auto chunk_g0 = [&](int c) {  // first arena group of chunk c of X_gi
-      const int g = xg0 + 2 * PF * c;
+      const int g = xg0 + GPS * PF * c;
       return g < SG ? g : g - SG;
     };
     // INT8: the accumulators start at the bias 128 * sum_k X[k][n] of their column,
@@ -903,7 +963,6 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       for (int t = fresh_tid(); t < TN; t += NTH) colsum[((gi + 2) % 3) * TN + t] = 0;
 
     Acc acc[RP][TNW];
-    Acc acc2[RP][TNW];  // FP32: the odd MFMAs of every step (mma_k)
     long held_rb0 = -1;  // final pass kept in acc until X_gi is dead
     for (long rb0 = (long)wave * RP * 32; rb0 < Mp; rb0 += PASS) {
 #pragma unroll
@@ -911,13 +970,10 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
 #pragma unroll
         for (int j = 0; j < TNW; ++j)
 #pragma unroll
-          for (int r = 0; r < 16; ++r) {
-            acc[i][j][r] = bias0[j];
-            acc2[i][j][r] = 0;
-          }
+          for (int r = 0; r < 16; ++r) acc[i][j][r] = bias0[j];
       const bool final_pass = rb0 + PASS >= Mp;
       const char *nxt = final_pass ? next_gemm : apk + pass_stride;
-      const unsigned after_lane = (final_pass && next_gemm) ? next_lane : alane;
+      const Lane after_lane = (final_pass && next_gemm) ? next_lane : alane;
       // With no next pass the prefetch re-reads this pass's first chunk: the loads
       // stay unconditional, so their wait counts are exact.
       const char *after = nxt ? nxt : apk;
@@ -934,7 +990,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
           if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
           else read_b(gn, 0, bq[0]);
           __builtin_amdgcn_sched_barrier(0);
-          mma_k<KIND, TNW>(bq[s & 1], a[s], acc, acc2);
+          mma_k<KIND, TNW>(bq[s & 1], a[s], acc);
           __builtin_amdgcn_sched_barrier(0);
         }
       };
@@ -943,7 +999,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       // loads issued in one burst per chunk stall the waves behind the vector-memory
       // queue, spread out they overlap the MFMAs. The loads are unconditional (also
       // in the skipped padding steps), so every fragment of the next chunk arrives.
-      auto run_chunk_il = [&](int c, const AFrag<KIND>(&a)[PF][RP], const char *nbase, unsigned nlane,
+      auto run_chunk_il = [&](int c, const AFrag<KIND>(&a)[PF][RP], const char *nbase, Lane nlane,
                               AFrag<KIND>(&na)[PF][RP]) {
         const int g0 = chunk_g0(c), gn = chunk_g0(c + 1 < nchunks ? c + 1 : 0);
 #pragma unroll
@@ -954,7 +1010,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
             if (s + 1 < PF) read_b(g0, s + 1, bq[(s + 1) & 1]);
             else read_b(gn, 0, bq[0]);
             __builtin_amdgcn_sched_barrier(0);
-            mma_k<KIND, TNW>(bq[s & 1], a[s], acc, acc2);
+            mma_k<KIND, TNW>(bq[s & 1], a[s], acc);
           }
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -982,14 +1038,6 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
       }
       apk = after;
       alane = after_lane;
-      if (KIND == K_F32) {
-#pragma unroll
-        for (int i = 0; i < RP; ++i)
-#pragma unroll
-          for (int j = 0; j < TNW; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] += acc2[i][j][r];
-      }
       if (final_pass && !last && ((P.hold >> gi) & 1)) held_rb0 = rb0;
       else store_rows_impl<KIND, TN, TNW, EPX>(P, gi, rb0, n0, arena, acc, csum_next, cb0);
     }

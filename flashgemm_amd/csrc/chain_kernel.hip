@@ -1022,7 +1022,7 @@ __device__ __forceinline__ void chain_tile(const ChainParams &P, const long n0, 
           run_chunk_il(c, fa[0], apk + (c + 1) * CHUNK, alane, fa[1]);
           run_chunk_il(c + 1, fa[1], c + 2 < nchunks ? apk + (c + 2) * CHUNK : after,
                        c + 2 < nchunks ? alane : after_lane, fa[0]);
-        } else {  // FP32 (4 MFMAs per fragment): one burst per chunk measured no slower
+        } else {  // FP32 (16 MFMAs per fragment): one burst per chunk measured faster
           load_chunk(apk + (c + 1) * CHUNK, alane, fa[1]);
           __builtin_amdgcn_sched_barrier(0);
           run_chunk(c, fa[0]);

@@ -17,7 +17,7 @@ enum Dtype : int { DT_F32 = 0, DT_BF16 = 1, DT_INT8 = 2 };
 
 // How an FP32 problem is computed.
 enum F32Mode : int {
-  F32_MFMA = 0,    // v_mfma_f32_32x32x2_f32, fp32 products and sums
+  F32_MFMA = 0,    // v_mfma_f32_16x16x4_f32, fp32 products and sums
   F32_BF16X3 = 1,  // x = hi + lo (bf16 pair); Ah.Bh + Ah.Bl + Al.Bh on bf16 MFMA
 };
 
